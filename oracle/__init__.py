@@ -1,0 +1,153 @@
+"""ctypes loader for the CPU oracle (oracle/libwforacle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker and the CPU baseline.  The product never imports it.
+The C source restates the algorithm of A001220/wieferich.c (see wf_oracle.h).
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libwforacle.so")
+MAXB = 64
+
+
+class Event(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("j", ctypes.c_uint32), ("p", ctypes.c_uint64),
+                ("q", ctypes.c_uint64), ("r_hi", ctypes.c_uint64), ("r_lo", ctypes.c_uint64)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("primes", ctypes.c_uint64), ("tests", ctypes.c_uint64), ("near", ctypes.c_uint64),
+                ("flt", ctypes.c_uint64), ("cks", ctypes.c_uint64 * MAXB), ("sumq", ctypes.c_uint64 * MAXB),
+                ("nev", ctypes.c_uint64), ("npsp", ctypes.c_uint64)]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.check_call(["make", "-s", "-B", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        u64, u32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64
+        P64 = ctypes.POINTER(u64)
+        L.wfo_mix64.restype = u64
+        L.wfo_mix64.argtypes = [u64]
+        L.wfo_is_prime64.restype = ctypes.c_int
+        L.wfo_is_prime64.argtypes = [u64]
+        L.wfo_sprp2.restype = ctypes.c_int
+        L.wfo_sprp2.argtypes = [u64]
+        L.wfo_fermat_quotient.restype = u64
+        L.wfo_fermat_quotient.argtypes = [u64, u32, P64, P64]
+        L.wfo_powmod_ref.restype = None
+        L.wfo_powmod_ref.argtypes = [u32, u64, u64, u64, P64, P64]
+        L.wfo_pctx.restype = None
+        L.wfo_pctx.argtypes = [u64, P64, P64, P64, P64]
+        for f in (L.wfo_msqr, L.wfo_redc):
+            f.restype = None
+            f.argtypes = [u64, u64, u64, P64, P64]
+        L.wfo_msmall.restype = None
+        L.wfo_msmall.argtypes = [u64, u64, u64, u64, P64, P64]
+        L.wfo_primes.restype = u64
+        L.wfo_primes.argtypes = [u64, u64, P64, u64, ctypes.c_int]
+        L.wfo_scan.restype = ctypes.c_int
+        L.wfo_scan.argtypes = [u64, u64, ctypes.POINTER(u32), ctypes.c_int, i64, ctypes.c_int,
+                               ctypes.POINTER(Result), ctypes.POINTER(Event), u64, ctypes.c_int, P64, u64]
+        _lib = L
+    return _lib
+
+
+def _hi(hi):
+    """2^64 is passed as 0."""
+    return 0 if hi == 1 << 64 else hi
+
+
+def mix64(z):
+    return lib().wfo_mix64(z)
+
+
+def is_prime64(n):
+    return bool(lib().wfo_is_prime64(n))
+
+
+def sprp2(n):
+    return bool(lib().wfo_sprp2(n))
+
+
+def fermat_quotient(p, b=2):
+    """(q, r) with r = b^(p-1) mod p^2 and q = (r - 1) p^-1 mod 2^64."""
+    hi, lo = ctypes.c_uint64(), ctypes.c_uint64()
+    q = lib().wfo_fermat_quotient(p, b, ctypes.byref(hi), ctypes.byref(lo))
+    return q, (hi.value << 64) | lo.value
+
+
+def powmod_ref(b, e, m):
+    hi, lo = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().wfo_powmod_ref(b, e, m >> 64, m & ((1 << 64) - 1), ctypes.byref(hi), ctypes.byref(lo))
+    return (hi.value << 64) | lo.value
+
+
+def pctx(p):
+    a, b, c, d = (ctypes.c_uint64() for _ in range(4))
+    lib().wfo_pctx(p, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+    return a.value, b.value, (c.value << 64) | d.value
+
+
+def _two(fn, *args):
+    hi, lo = ctypes.c_uint64(), ctypes.c_uint64()
+    fn(*args, ctypes.byref(hi), ctypes.byref(lo))
+    return (hi.value << 64) | lo.value
+
+
+def msqr(p, x):
+    return _two(lib().wfo_msqr, p, x >> 64, x & ((1 << 64) - 1))
+
+
+def msmall(p, x, c):
+    return _two(lib().wfo_msmall, p, x >> 64, x & ((1 << 64) - 1), c)
+
+
+def redc(p, x):
+    return _two(lib().wfo_redc, p, x >> 64, x & ((1 << 64) - 1))
+
+
+def primes(lo, hi, method=0, cap=None):
+    """List of primes in [lo, hi) (hi may be 2^64)."""
+    L = lib()
+    n = L.wfo_primes(lo, _hi(hi), None, 0, method)
+    buf = (ctypes.c_uint64 * max(n, 1))()
+    L.wfo_primes(lo, _hi(hi), buf, n, method)
+    return list(buf[:n])
+
+
+def scan(lo, hi, bases=(2,), near_T=0, threads=None, ev_cap=1 << 16, psp=False, psp_cap=1 << 16):
+    """Scan [lo, hi): dict with primes, tests, near, flt, cks/sumq per base, events, psp list."""
+    L = lib()
+    bases = sorted(set(bases))
+    nb = len(bases)
+    threads = threads or min(os.cpu_count() or 1, 16)
+    res = Result()
+    ev = (Event * max(ev_cap, 1))()
+    pspbuf = (ctypes.c_uint64 * max(psp_cap, 1))()
+    barr = (ctypes.c_uint32 * nb)(*bases)
+    rc = L.wfo_scan(lo, _hi(hi), barr, nb, near_T, threads, ctypes.byref(res), ev, ev_cap,
+                    1 if psp else 0, pspbuf, psp_cap)
+    if rc != 0:
+        raise ValueError("wfo_scan: bad arguments")
+    events = [(e.type, bases[e.j], e.p, e.q, (e.r_hi << 64) | e.r_lo) for e in ev[:min(res.nev, ev_cap)]]
+    return {
+        "primes": res.primes, "tests": res.tests, "near": res.near, "flt": res.flt,
+        "cks": {b: res.cks[j] for j, b in enumerate(bases)},
+        "sumq": {b: res.sumq[j] for j, b in enumerate(bases)},
+        "nev": res.nev, "events": events,
+        "npsp": res.npsp, "psp": list(pspbuf[:min(res.npsp, psp_cap)]),
+    }
