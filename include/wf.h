@@ -1,0 +1,127 @@
+/*
+ * wf.h -- C ABI of the MI355X Wieferich search engine (libwf.so).
+ *
+ * The engine runs the search loop of A001220/wieferich.c (jeffsponaugle/Mathpub) on one
+ * AMD MI355X: a wheel-30 segmented sieve over [lo, hi) on the GPU, then for every odd
+ * prime p the base-2 Fermat quotient q = (2^(p-1) mod p^2 - 1) / p, its checksum term
+ * mix64(q ^ p * 0x9E3779B97F4A7C15), the FLT check q < p and the FOUND / NEAR events.
+ * Results are bit-identical to the reference CPU tool.
+ *
+ * The reference has no plugin API; the entry points below replace its two internal seams
+ * (the per-chunk scan and the prime source):
+ *   wf_scan        scan_chunk()          A001220/wieferich.c:1129-1177 (with the prime source,
+ *                  primesieve_jump_to/next_prime, :1138-1141) and the GPU tool's
+ *                  sieve_chunk/enqueue_chunk/fold_chunk, A001220/cuda/wieferich_cuda.cu:1090-1271
+ *   wf_scan_batch  (no counterpart: many uneven ranges in one launch)
+ *   wf_quotients   fermat_quotient()     A001220/wieferich.c:939-949, GPU k_quot
+ *                  A001220/cuda/wieferich_cuda.cu:976-983 (check / selftest)
+ *   wf_primes      primesieve_jump_to / primesieve_next_prime, A001220/wieferich.c:1138-1141
+ *                  (the prime set of [lo, hi), for tests and the sieve benchmark)
+ *   wf_serve_*     (no counterpart: a device-resident request loop)
+ *
+ * Conventions (as wieferich.c): return 0 on success or a negative WF_E_* code
+ * (wf_strerror names it; the host program prints it and exits with status 1, as die()
+ * does, wieferich.c:271-280).  A range is [lo, hi) with hi == 0 meaning 2^64.  The
+ * prime 2 is counted but not tested (wieferich.c:1142-1146).  An engine belongs to one
+ * host thread; calls on one engine must not overlap.  The caller owns every output
+ * buffer; the engine owns its device memory.  Events beyond ev_cap are counted in nev
+ * and never silently dropped (the caller sees nev > ev_cap).
+ */
+#ifndef WF_H
+#define WF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_MAXB 64
+
+enum {
+    WF_OK = 0,
+    WF_E_ARG = -1,          /* bad argument (range, sizes, NULL pointer) */
+    WF_E_DEVICE = -2,       /* no GPU, or a HIP runtime call failed */
+    WF_E_NOMEM = -3,        /* device or host allocation failed */
+    WF_E_UNSUPPORTED = -4,  /* a base other than 2, or a feature not built */
+    WF_E_OVERFLOW = -5,     /* an internal buffer bound was exceeded (bug guard) */
+    WF_E_BUSY = -6,         /* serving: no free slot / HBM budget */
+};
+
+/* Event types, the reference's result lines (wieferich.c:1009, :1060, :1071). */
+enum { WF_EV_FOUND = 0, WF_EV_NEAR = 1, WF_EV_FLT = 2 };
+
+typedef struct {
+    uint32_t type, j;       /* type, base index */
+    uint64_t p, q;          /* prime and Fermat quotient (q >= p for FLT) */
+    uint64_t r_hi, r_lo;    /* b^(p-1) mod p^2 */
+} wf_event;
+
+/* Totals of one scanned range: the reference's slot_t (wieferich.c:956-960) plus the
+ * sum of the quotients and the event / pseudoprime counts. */
+typedef struct {
+    uint64_t primes, tests, near, flt;
+    uint64_t cks[WF_MAXB];
+    uint64_t sumq[WF_MAXB];
+    uint64_t nev;           /* events of this range (all of them, even beyond ev_cap) */
+    uint64_t npsp;          /* odd composites that are base-2 strong probable primes */
+} wf_result;
+
+typedef struct { uint64_t lo, hi; } wf_range;   /* [lo, hi), hi == 0 means 2^64 */
+
+/* Tuning knobs; a zero field means "engine default". */
+typedef struct {
+    uint32_t seg_bytes;     /* LDS sieve segment, 16384 .. 65536 (a power of two) */
+    uint32_t sieve_block;   /* threads per sieve workgroup (256 .. 1024) */
+    uint32_t test_block;    /* threads per test workgroup (64 .. 1024) */
+    uint32_t arith;         /* 0 = fastest exact path, 1 = force 32-bit Montgomery */
+    uint64_t chunk;         /* numbers per device pass of wf_scan (memory bound) */
+} wf_config;
+
+/* Engine life cycle (the reference's gpu_init, wieferich_cuda.cu:1570-1577). */
+int  wf_open(int device, const wf_config *cfg, struct wf_engine **out);
+void wf_close(struct wf_engine *e);
+const char *wf_strerror(int rc);
+/* Device name, CU count and the arithmetic path, for status lines. */
+int  wf_info(struct wf_engine *e, char *buf, int len);
+
+/* Bases of the run, ascending and distinct.  The GPU kernels implement base 2. */
+int  wf_set_bases(struct wf_engine *e, const uint32_t *bases, int nb);
+
+/* Scan one range: totals into *out, events sorted by (p, j) into ev[0..ev_cap).
+ * near_T > 0 reports 0 < |A| <= near_T (wieferich.c:1065-1073).  want_psp != 0 also
+ * lists the odd composites of the range that are base-2 strong probable primes, sorted,
+ * into psp[0..psp_cap) (out->npsp counts all of them). */
+int  wf_scan(struct wf_engine *e, uint64_t lo, uint64_t hi, int64_t near_T, wf_result *out,
+             wf_event *ev, uint64_t ev_cap, int want_psp, uint64_t *psp, uint64_t psp_cap);
+
+/* Scan n ranges in one device pass: per_range[i] receives range i's totals; events of
+ * all ranges go to ev (sorted by (p, j); a range is found by p). */
+int  wf_scan_batch(struct wf_engine *e, const wf_range *r, uint32_t n, int64_t near_T,
+                   wf_result *per_range, wf_event *ev, uint64_t ev_cap);
+
+/* Fermat quotients of arbitrary odd p (prime or not) for base 2: r = 2^(p-1) mod p^2,
+ * q = (r - 1) p^-1 mod 2^64, as fermat_quotient() (wieferich.c:939-949). */
+int  wf_quotients(struct wf_engine *e, const uint64_t *p, uint64_t *q, uint64_t *r_hi,
+                  uint64_t *r_lo, uint64_t n);
+
+/* The primes of [lo, hi) in ascending order into out[0..cap); *count = how many exist. */
+int  wf_primes(struct wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *out, uint64_t cap,
+               uint64_t *count);
+
+/* Serving: a device-resident request loop with continuous batching.  submit admits a
+ * range when a slot and the HBM budget allow (else WF_E_BUSY); poll returns its result
+ * once done (*done = 1).  Results equal wf_scan of the same range. */
+int  wf_serve_start(struct wf_engine *e, uint32_t slots, uint64_t hbm_budget);
+int  wf_serve_submit(struct wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *ticket);
+int  wf_serve_poll(struct wf_engine *e, uint64_t ticket, wf_result *out, int *done);
+int  wf_serve_stop(struct wf_engine *e);
+
+/* Timings of the last wf_scan / wf_scan_batch: device milliseconds of the sieve and
+ * the test kernels (HIP events on the engine's stream), primes tested. */
+typedef struct { double sieve_ms, test_ms, total_ms; uint64_t tested; uint64_t launches; } wf_timing;
+int  wf_last_timing(struct wf_engine *e, wf_timing *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

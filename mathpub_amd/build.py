@@ -1,0 +1,49 @@
+"""Builds the gfx950 HIP engine (libwf.so) and the C host program (wf) in-tree.
+
+hipcc cross-compiles for gfx950 without a GPU, so this runs on any build machine.
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+INC = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libwf.so")
+HOST = os.path.join(PKG, "wf")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    # packed FP32 instructions are switched off for every HIP file of this project
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build_lib():
+    src = os.path.join(PKG, "csrc", "wf_engine.hip")
+    tmp = LIB + ".tmp"
+    run([HIPCC] + HIP_FLAGS + ["-I" + INC, "-I" + os.path.join(PKG, "csrc"), "-o", tmp, src])
+    os.replace(tmp, LIB)
+
+
+def build_host():
+    src = os.path.join(PKG, "host", "wf.c")
+    if not os.path.exists(src):
+        return
+    run(["gcc", "-O2", "-std=gnu11", "-Wall", "-Wextra", "-pthread", "-I" + INC, "-o", HOST, src,
+         "-L" + PKG, "-lwf", "-Wl,-rpath,$ORIGIN", "-lm"])
+
+
+def main():
+    build_lib()
+    build_host()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,300 @@
+// wf_arith.h -- exact modular arithmetic mod N = p^2 for the base-2 Fermat quotient on
+// gfx950 (CDNA4), one prime per lane.
+//
+// Two exact implementations of r = 2^(p-1) mod p^2 (A001220/wieferich.c:893-949 computes
+// the same residue with 2 x 64-bit Montgomery limbs and a 5-bit window):
+//
+//  * fp64_fermat2: p < 2^50.  The residue is held in p-adic digits x = a + b p with a, b
+//    signed integers stored exactly in doubles.  A squaring needs a^2 mod p, floor(a^2/p)
+//    and 2ab mod p; each product is split exactly by FMA (h = x*y, l = fma(x, y, -h)) and
+//    reduced with a rounded quotient k = rint(h / p) and an exact FMA remainder.  gfx950
+//    issues FP64 FMA at the full VALU rate, four times the 32-bit integer multiply rate.
+//
+//  * mont_fermat2<L, FULL>: any odd p < 2^64.  L 32-bit limbs, R = 2^(32 L), products
+//    from v_mad_u64_u32 / v_mul_lo_u32 chains (operand scanning), lazy values in [0, 2N)
+//    while N < R/4 and full reduction with the carry bit (FULL) when N may reach 2^128.
+//    The base-2 ladder is left to right: square, then double when the exponent bit is 1.
+//
+// Both return the residue; the caller forms q = (r - 1) p^-1 mod 2^64 and the FLT check
+// q < p exactly as wieferich.c:1051-1063 does.
+#pragma once
+#include <stdint.h>
+#ifdef WF_HOST_EMU
+// Host build of the same arithmetic for CPU tests (tests/emu): IEEE-754 doubles with FMA
+// give the same results as the GPU's v_fma_f64 / v_rndne_f64 / v_ldexp_f64.
+#include <cmath>
+#define __device__
+#define __forceinline__ inline
+static inline uint64_t __umul64hi(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) >> 64); }
+#define __builtin_fma(a, b, c) std::fma(a, b, c)
+#define __builtin_rint(a) std::rint(a)
+#define __builtin_ldexp(a, d) std::ldexp(a, d)
+#define __builtin_floor(a) std::floor(a)
+#else
+#include <hip/hip_runtime.h>
+#endif
+
+namespace wf {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+// p^-1 mod 2^64 by Newton iteration from 3p XOR 2 (correct to 5 bits), 5->80 bits.
+__device__ __forceinline__ u64 inv64(u64 p) {
+  u64 x = (3 * p) ^ 2;
+  x *= 2 - p * x;
+  x *= 2 - p * x;
+  x *= 2 - p * x;
+  x *= 2 - p * x;
+  return x;
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 p-adic path, p < 2^50
+// ---------------------------------------------------------------------------------
+//
+// Invariants (u = 2^-53, p < 2^50, all values integers held exactly):
+//   |a| <= 0.667 p (0.5 p right after a window step), |b| <= 1.9 p.
+// Squaring, x^2 = a^2 + 2ab p (mod p^2):
+//   a^2 = k1 p + a'      k1 = rint(fl(a*a) * ip): |k1 - a^2/p| <= 0.5 + 0.375 alpha^2,
+//                       so alpha' = 0.5 + 0.375 alpha^2, fixed point 0.667;
+//   ab  = (k2 p + e)/2   k2 = rint(fl(a*b) * 2ip), e/2 = fma(-k2, p/2, h) + l exact
+//                       (a multiple of 1/2 below 2^50 in magnitude);
+//   b'  = k1 + e        |b'| <= alpha^2 p + 1 + (0.5 + 0.75 alpha beta) p, fixed point 1.9 p.
+// Every intermediate stays below 2^52 in magnitude, so each FMA result is exact.
+struct F64Mod {
+  double p, ip, ip2, ph;   // p, 1/p, 2/p, p/2
+};
+
+__device__ __forceinline__ void f64_sqr(double& a, double& b, const F64Mod& m) {
+  const double h1 = a * a;
+  const double l1 = __builtin_fma(a, a, -h1);
+  const double k1 = __builtin_rint(h1 * m.ip);
+  const double a1 = __builtin_fma(-k1, m.p, h1) + l1;
+  const double h2 = a * b;
+  const double l2 = __builtin_fma(a, b, -h2);
+  const double k2 = __builtin_rint(h2 * m.ip2);
+  const double e2 = __builtin_fma(-k2, m.ph, h2) + l2;  // (2ab - k2 p) / 2
+  b = __builtin_fma(2.0, e2, k1);
+  a = a1;
+}
+
+// x *= 2^d for 0 <= d <= 31: a 2^d and b 2^d are exact (power-of-two scaling), each is
+// reduced by an exact FMA remainder, and the carry ka moves from a into b.
+__device__ __forceinline__ void f64_mul_pow2(double& a, double& b, int d, const F64Mod& m) {
+  const double A = __builtin_ldexp(a, d);
+  const double ka = __builtin_rint(A * m.ip);
+  a = __builtin_fma(-ka, m.p, A);
+  const double B = __builtin_ldexp(b, d);
+  const double kb = __builtin_rint(B * m.ip);
+  const double b1 = __builtin_fma(-kb, m.p, B) + ka;
+  const double kc = __builtin_rint(b1 * m.ip);
+  b = __builtin_fma(-kc, m.p, b1);
+}
+
+// r = 2^e mod p^2 as (a, b) normalised to 0 <= a, b < p (r = a + b p).  ebits = the
+// number of bits to process (the wave's longest exponent), W = window width.
+template <int W>
+__device__ __forceinline__ void fp64_pow2(u64 p, u64 e, int ebits, double& ra, double& rb) {
+  F64Mod m;
+  m.p = (double)p;
+  m.ip = 1.0 / m.p;
+  m.ip2 = 2.0 * m.ip;
+  m.ph = 0.5 * m.p;
+  const int nw = (ebits + W - 1) / W;
+  int sh = (nw - 1) * W;
+  double a = 1.0, b = 0.0;
+  f64_mul_pow2(a, b, (int)(e >> sh), m);
+  for (sh -= W; sh >= 0; sh -= W) {
+#pragma unroll
+    for (int s = 0; s < W; s++) f64_sqr(a, b, m);
+    f64_mul_pow2(a, b, (int)((e >> sh) & ((1u << W) - 1)), m);
+  }
+  // normalise: a in [0, p), then b in [0, p)
+  if (a < 0.0) { a += m.p; b -= 1.0; }
+  else if (a >= m.p) { a -= m.p; b += 1.0; }
+  const double kb = __builtin_floor(b * m.ip);
+  b = __builtin_fma(-kb, m.p, b);
+  if (b < 0.0) b += m.p;
+  else if (b >= m.p) b -= m.p;
+  ra = a;
+  rb = b;
+}
+
+// ---------------------------------------------------------------------------------
+// 32-bit limb Montgomery path
+// ---------------------------------------------------------------------------------
+
+// Montgomery squaring t = x^2 / R mod N (R = 2^(32L)), ninv = -N^-1 mod 2^32.
+// Lazy (FULL = false): x < 2N, N < R/4 gives a result < 2N and no final subtraction.
+// FULL: x < N < R; the result (< 2N < 2^(32L+1)) keeps its carry bit and is reduced.
+template <int L, bool FULL>
+__device__ __forceinline__ void mont_sqr(u32 (&x)[L], const u32 (&n)[L], u32 ninv) {
+  u32 t[2 * L];
+#pragma unroll
+  for (int k = 0; k < 2 * L; k++) t[k] = 0;
+  // cross products x_i x_j, i < j
+#pragma unroll
+  for (int i = 0; i < L - 1; i++) {
+    u64 c = 0;
+#pragma unroll
+    for (int j = i + 1; j < L; j++) {
+      c = (u64)x[i] * x[j] + t[i + j] + (c >> 32);
+      t[i + j] = (u32)c;
+    }
+    t[i + L] = (u32)(c >> 32);
+  }
+  // double them
+#pragma unroll
+  for (int k = 2 * L - 1; k > 0; k--) t[k] = (t[k] << 1) | (t[k - 1] >> 31);
+  t[0] <<= 1;
+  // add the squares x_i^2 at word 2i
+  {
+    u64 c = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+      const u64 sq = (u64)x[i] * x[i];
+      c = (u64)t[2 * i] + (u32)sq + (c >> 32);
+      t[2 * i] = (u32)c;
+      c = (u64)t[2 * i + 1] + (u32)(sq >> 32) + (c >> 32);
+      t[2 * i + 1] = (u32)c;
+    }
+  }
+  // REDC: clear one low word per row; 'hi' is the carry pending into word i + L + 1
+  u32 hi = 0;
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    const u32 mq = t[i] * ninv;
+    u64 c = 0;
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+      c = (u64)mq * n[j] + t[i + j] + (c >> 32);
+      t[i + j] = (u32)c;
+    }
+    const u64 s = (u64)t[i + L] + (c >> 32) + hi;
+    t[i + L] = (u32)s;
+    hi = (u32)(s >> 32);
+  }
+#pragma unroll
+  for (int i = 0; i < L; i++) x[i] = t[L + i];
+  if (FULL) {
+    // result = hi 2^(32L) + x < 2N: subtract N once if it is >= N
+    u32 d[L];
+    u64 br = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+      const u64 s = (u64)x[i] - n[i] - br;
+      d[i] = (u32)s;
+      br = (s >> 32) & 1;
+    }
+    const bool ge = hi || !br;
+#pragma unroll
+    for (int i = 0; i < L; i++) x[i] = ge ? d[i] : x[i];
+  }
+}
+
+// x = 2x mod M (M = 2N in the lazy domain [0, 2N), M = N for FULL with carry-out).
+template <int L>
+__device__ __forceinline__ void mont_dbl(u32 (&x)[L], const u32 (&M)[L]) {
+  u32 y[L];
+  const u32 top = x[L - 1] >> 31;
+#pragma unroll
+  for (int k = L - 1; k > 0; k--) y[k] = (x[k] << 1) | (x[k - 1] >> 31);
+  y[0] = x[0] << 1;
+  u32 d[L];
+  u64 br = 0;
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    const u64 s = (u64)y[i] - M[i] - br;
+    d[i] = (u32)s;
+    br = (s >> 32) & 1;
+  }
+  const bool ge = top || !br;
+#pragma unroll
+  for (int i = 0; i < L; i++) x[i] = ge ? d[i] : y[i];
+}
+
+// REDC of a single-width value x < 2N: x / R mod N, fully reduced.
+template <int L>
+__device__ __forceinline__ void mont_out(u32 (&x)[L], const u32 (&n)[L], u32 ninv) {
+  u32 t[2 * L];
+#pragma unroll
+  for (int k = 0; k < L; k++) { t[k] = x[k]; t[L + k] = 0; }
+  u32 hi = 0;
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    const u32 mq = t[i] * ninv;
+    u64 c = 0;
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+      c = (u64)mq * n[j] + t[i + j] + (c >> 32);
+      t[i + j] = (u32)c;
+    }
+    const u64 s = (u64)t[i + L] + (c >> 32) + hi;
+    t[i + L] = (u32)s;
+    hi = (u32)(s >> 32);
+  }
+  u32 d[L];
+  u64 br = 0;
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    x[i] = t[L + i];
+    const u64 s = (u64)x[i] - n[i] - br;
+    d[i] = (u32)s;
+    br = (s >> 32) & 1;
+  }
+  const bool ge = hi || !br;
+#pragma unroll
+  for (int i = 0; i < L; i++) x[i] = ge ? d[i] : x[i];
+}
+
+// r = 2^e mod N through the L-limb Montgomery ladder; r has L limbs (N = p^2).
+// ebits = the wave's longest exponent length (shorter exponents see leading zeros).
+template <int L, bool FULL>
+__device__ __forceinline__ void mont_pow2(u64 p, u64 e, int ebits, u32 (&r)[L]) {
+  // N = p^2 as L limbs
+  u32 n[L];
+  {
+    const u64 lo = p * p;
+    const u64 hi = __umul64hi(p, p);
+    const u32 w4[4] = {(u32)lo, (u32)(lo >> 32), (u32)hi, (u32)(hi >> 32)};
+#pragma unroll
+    for (int i = 0; i < L; i++) n[i] = i < 4 ? w4[i] : 0u;
+  }
+  const u64 pinv = inv64(p);
+  const u32 ninv = 0u - (u32)pinv * (u32)pinv;
+  // M = modulus of the doubling: 2N (lazy) or N (full)
+  u32 M[L];
+  {
+    u32 c = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+      M[i] = FULL ? n[i] : ((n[i] << 1) | c);
+      c = n[i] >> 31;
+    }
+  }
+  // x = 2^(bits(N) - 1) < N, then doublings mod N up to 2^(32L + 1) = Montgomery form of 2
+  int topw = L - 1;
+  while (topw > 0 && n[topw] == 0) topw--;
+  const int nb = 32 * topw + (32 - __builtin_clz(n[topw]));  // bits(N)
+  u32 x[L];
+#pragma unroll
+  for (int i = 0; i < L; i++) x[i] = 0;
+  const int hb = nb - 1;
+#pragma unroll
+  for (int i = 0; i < L; i++) x[i] = (i == (hb >> 5)) ? (1u << (hb & 31)) : 0u;
+  for (int k = hb; k < 32 * L + 1; k++) mont_dbl<L>(x, n);   // reduce mod N while building
+  // ladder over the exponent bits below the top one
+  for (int i = ebits - 2; i >= 0; i--) {
+    const bool lead = i >= 63 - (int)__builtin_clzll(e);    // above this lane's top bit
+    if (!lead) {
+      mont_sqr<L, FULL>(x, n, ninv);
+      if ((e >> i) & 1) mont_dbl<L>(x, M);
+    }
+  }
+  mont_out<L>(x, n, ninv);
+#pragma unroll
+  for (int i = 0; i < L; i++) r[i] = x[i];
+}
+
+}  // namespace wf

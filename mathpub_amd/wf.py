@@ -1,0 +1,178 @@
+"""ctypes mirror of include/wf.h.
+
+Names follow the reference's domain: a scan of [lo, hi) returns the totals of
+A001220/wieferich.c's slot_t (primes, tests, near, flt, per-base checksums) plus the
+sum of the quotients, and FOUND / NEAR / FLT-ERROR events sorted by (p, base).
+"""
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(PKG, "libwf.so")
+MAXB = 64
+EV_FOUND, EV_NEAR, EV_FLT = 0, 1, 2
+
+u64, u32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64
+
+
+class Event(ctypes.Structure):
+    _fields_ = [("type", u32), ("j", u32), ("p", u64), ("q", u64), ("r_hi", u64), ("r_lo", u64)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("primes", u64), ("tests", u64), ("near", u64), ("flt", u64), ("cks", u64 * MAXB),
+                ("sumq", u64 * MAXB), ("nev", u64), ("npsp", u64)]
+
+
+class Range(ctypes.Structure):
+    _fields_ = [("lo", u64), ("hi", u64)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("seg_bytes", u32), ("sieve_block", u32), ("test_block", u32), ("arith", u32), ("chunk", u64)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("sieve_ms", ctypes.c_double), ("test_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("tested", u64), ("launches", u64)]
+
+
+class WFError(RuntimeError):
+    def __init__(self, rc, what):
+        super().__init__("%s: %s (%d)" % (what, _lib.wf_strerror(rc).decode(), rc))
+        self.rc = rc
+
+
+_lib = None
+
+
+def load():
+    """Load libwf.so (built by mathpub_amd/build.py); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        raise RuntimeError("libwf.so is not built: run __graft_entry__.build() (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB)
+    P64 = ctypes.POINTER(u64)
+    L.wf_strerror.restype = ctypes.c_char_p
+    L.wf_strerror.argtypes = [ctypes.c_int]
+    L.wf_open.argtypes = [ctypes.c_int, ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_void_p)]
+    L.wf_close.argtypes = [ctypes.c_void_p]
+    L.wf_close.restype = None
+    L.wf_info.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+    L.wf_set_bases.argtypes = [ctypes.c_void_p, ctypes.POINTER(u32), ctypes.c_int]
+    L.wf_scan.argtypes = [ctypes.c_void_p, u64, u64, i64, ctypes.POINTER(Result), ctypes.POINTER(Event), u64,
+                          ctypes.c_int, P64, u64]
+    L.wf_scan_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Range), u32, i64, ctypes.POINTER(Result),
+                                ctypes.POINTER(Event), u64]
+    L.wf_quotients.argtypes = [ctypes.c_void_p, P64, P64, P64, P64, u64]
+    L.wf_primes.argtypes = [ctypes.c_void_p, u64, u64, P64, u64, P64]
+    L.wf_last_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(Timing)]
+    L.wf_serve_start.argtypes = [ctypes.c_void_p, u32, u64]
+    L.wf_serve_submit.argtypes = [ctypes.c_void_p, u64, u64, P64]
+    L.wf_serve_poll.argtypes = [ctypes.c_void_p, u64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_int)]
+    L.wf_serve_stop.argtypes = [ctypes.c_void_p]
+    for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_primes",
+              "wf_last_timing", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
+        getattr(L, f).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _hi(hi):
+    if hi == 1 << 64:
+        return 0
+    if not 0 <= hi < 1 << 64:
+        raise ValueError("hi out of range")
+    return hi
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise WFError(rc, what)
+
+
+def result_dict(res, events=None):
+    return {"primes": res.primes, "tests": res.tests, "near": res.near, "flt": res.flt,
+            "cks": res.cks[0], "sumq": res.sumq[0], "nev": res.nev, "npsp": res.npsp,
+            "events": events if events is not None else []}
+
+
+class Engine:
+    """One GPU engine (wf_open); not thread-safe, like the C object."""
+
+    def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0):
+        L = load()
+        cfg = Config(seg_bytes, sieve_block, 0, arith, chunk)
+        h = ctypes.c_void_p()
+        _check(L.wf_open(device, ctypes.byref(cfg), ctypes.byref(h)), "wf_open")
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if self._h:
+            self._L.wf_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        buf = ctypes.create_string_buffer(256)
+        _check(self._L.wf_info(self._h, buf, 256), "wf_info")
+        return buf.value.decode()
+
+    def set_bases(self, bases):
+        arr = (u32 * len(bases))(*bases)
+        _check(self._L.wf_set_bases(self._h, arr, len(bases)), "wf_set_bases")
+
+    def scan(self, lo, hi, near_T=0, ev_cap=1 << 16, psp=False, psp_cap=1 << 16):
+        res = Result()
+        ev = (Event * max(ev_cap, 1))()
+        pbuf = (u64 * max(psp_cap, 1))()
+        _check(self._L.wf_scan(self._h, lo, _hi(hi), near_T, ctypes.byref(res), ev, ev_cap, 1 if psp else 0,
+                               pbuf, psp_cap), "wf_scan")
+        n = min(res.nev, ev_cap)
+        out = result_dict(res, [(e.type, e.p, e.q, (e.r_hi << 64) | e.r_lo) for e in ev[:n]])
+        out["psp"] = list(pbuf[:min(res.npsp, psp_cap)]) if psp else None
+        return out
+
+    def scan_batch(self, ranges, near_T=0, ev_cap=1 << 16):
+        n = len(ranges)
+        rr = (Range * n)(*[Range(lo, _hi(hi)) for lo, hi in ranges])
+        res = (Result * n)()
+        ev = (Event * max(ev_cap, 1))()
+        _check(self._L.wf_scan_batch(self._h, rr, n, near_T, res, ev, ev_cap), "wf_scan_batch")
+        total_ev = sum(r.nev for r in res)
+        events = [(e.type, e.p, e.q, (e.r_hi << 64) | e.r_lo) for e in ev[:min(total_ev, ev_cap)]]
+        return [result_dict(r) for r in res], events
+
+    def quotients(self, ps):
+        n = len(ps)
+        arr = (u64 * n)(*ps)
+        q, rh, rl = (u64 * n)(), (u64 * n)(), (u64 * n)()
+        _check(self._L.wf_quotients(self._h, arr, q, rh, rl, n), "wf_quotients")
+        return [(q[i], (rh[i] << 64) | rl[i]) for i in range(n)]
+
+    def primes(self, lo, hi):
+        cnt = u64()
+        _check(self._L.wf_primes(self._h, lo, _hi(hi), None, 0, ctypes.byref(cnt)), "wf_primes")
+        buf = (u64 * max(cnt.value, 1))()
+        _check(self._L.wf_primes(self._h, lo, _hi(hi), buf, cnt.value, ctypes.byref(cnt)), "wf_primes")
+        return list(buf[:cnt.value])
+
+    def last_timing(self):
+        t = Timing()
+        _check(self._L.wf_last_timing(self._h, ctypes.byref(t)), "wf_last_timing")
+        return {"sieve_ms": t.sieve_ms, "test_ms": t.test_ms, "total_ms": t.total_ms, "tested": t.tested,
+                "launches": t.launches}

@@ -1,0 +1,32 @@
+// Host build of mathpub_amd/csrc/wf_arith.h for CPU tests of the kernel arithmetic.
+#define WF_HOST_EMU 1
+#include "wf_arith.h"
+
+using namespace wf;
+
+extern "C" {
+
+// 2^(p-1) mod p^2 through the FP64 p-adic path (p < 2^50): r = hi:lo
+void emu_fp64(uint64_t p, uint64_t* hi, uint64_t* lo) {
+  const uint64_t e = p - 1;
+  const int eb = 64 - __builtin_clzll(e);
+  double a, b;
+  fp64_pow2<5>(p, e, eb, a, b);
+  const unsigned __int128 r = (unsigned __int128)(uint64_t)b * p + (uint64_t)a;
+  *hi = (uint64_t)(r >> 64);
+  *lo = (uint64_t)r;
+}
+
+// the same through the L-limb Montgomery ladder; full = carry-keeping variant
+void emu_mont(int L, int full, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
+  const uint64_t e = p - 1;
+  uint32_t x[4] = {0, 0, 0, 0};
+  if (L == 2) { uint32_t y[2]; mont_pow2<2, false>(p, e, ebits, y); x[0] = y[0]; x[1] = y[1]; }
+  else if (L == 3) { uint32_t y[3]; mont_pow2<3, false>(p, e, ebits, y); x[0] = y[0]; x[1] = y[1]; x[2] = y[2]; }
+  else if (!full) { mont_pow2<4, false>(p, e, ebits, x); }
+  else { mont_pow2<4, true>(p, e, ebits, x); }
+  *lo = (uint64_t)x[0] | ((uint64_t)x[1] << 32);
+  *hi = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
+}
+
+}

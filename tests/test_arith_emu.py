@@ -1,0 +1,68 @@
+"""The kernel arithmetic of wf_arith.h, compiled for the host (IEEE doubles with FMA
+behave as the GPU's), against Python's pow: FP64 p-adic path and every Montgomery width,
+including exponents padded with leading zeros as in a wavefront of mixed lengths."""
+import ctypes
+import os
+import random
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SO = os.path.join(HERE, "emu", "_build", "arith_emu.so")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    os.makedirs(os.path.dirname(SO), exist_ok=True)
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-mfma", "-shared", "-fPIC",
+                           "-I" + os.path.join(ROOT, "mathpub_amd", "csrc"), "-o", SO,
+                           os.path.join(HERE, "emu", "arith_emu.cpp")])
+    L = ctypes.CDLL(SO)
+    P = ctypes.POINTER(ctypes.c_uint64)
+    L.emu_fp64.argtypes = [ctypes.c_uint64, P, P]
+    L.emu_mont.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, P, P]
+    return L
+
+
+def call(fn, *args):
+    hi, lo = ctypes.c_uint64(), ctypes.c_uint64()
+    fn(*args, ctypes.byref(hi), ctypes.byref(lo))
+    return (hi.value << 64) | lo.value
+
+
+def samples(lo_bits, hi_bits, n, seed):
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        bits = rng.randint(lo_bits, hi_bits)
+        p = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
+        out.append(p)
+    return out
+
+
+def test_fp64_padic(emu):
+    ps = [3, 5, 7, 9, 1093, 3511, (1 << 50) - 1, (1 << 50) - 27, (1 << 49) + 1, 10 ** 15 + 37] + \
+        samples(2, 50, 20000, 1)
+    for p in ps:
+        if p < 3:
+            continue
+        assert call(emu.emu_fp64, p) == pow(2, p - 1, p * p), p
+
+
+@pytest.mark.parametrize("L,full,lo,hi", [(2, 0, 2, 31), (3, 0, 32, 47), (4, 0, 48, 63), (4, 1, 64, 64),
+                                          (4, 1, 63, 64)])
+def test_montgomery(emu, L, full, lo, hi):
+    ps = samples(lo, hi, 5000, L * 10 + full)
+    if L == 4 and full:
+        ps += [(1 << 64) - 1, (1 << 64) - 59, (1 << 63) + 1]
+    if L == 2:
+        ps += [3, 5, 7, 9, (1 << 31) - 1]
+    for p in ps:
+        if p < 3:
+            continue
+        eb = (p - 1).bit_length()
+        assert call(emu.emu_mont, L, full, p, eb) == pow(2, p - 1, p * p), (L, p)
+        # a longer wave-wide exponent length only adds skipped leading bits
+        assert call(emu.emu_mont, L, full, p, min(64, eb + 3)) == pow(2, p - 1, p * p), (L, p)

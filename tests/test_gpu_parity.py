@@ -1,0 +1,107 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the golden vectors.
+
+Bar: bit-exact prime sets, counts, checksums, sums of q, near lists and solutions.
+"""
+import json
+import os
+import random
+import sys
+
+import pytest
+
+import oracle
+from mathpub_amd import wf
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+WINS = json.load(open(os.path.join(G, "base2_windows.json")))
+FACTS = json.load(open(os.path.join(G, "reference_facts.json")))
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module", params=[0, 1], ids=["fast", "int32"])
+def eng(request):
+    e = wf.Engine(arith=request.param)
+    yield e
+    e.close()
+
+
+def near_of(events):
+    return [[p, (q if q <= p // 2 else q - p)] for t, p, q, r in events if t == wf.EV_NEAR]
+
+
+@pytest.mark.parametrize("lo,hi", [(0, 1000), (0, 3), (2, 3), (3, 4), (7, 8), (1, 120), (29, 31), (30, 31),
+                                   (999983, 1100000), (10 ** 15, 10 ** 15 + 300000),
+                                   ((1 << 64) - 200000, 1 << 64), ((1 << 32) - 1000, (1 << 32) + 1000)])
+def test_primes_match_oracle(eng, lo, hi):
+    say("primes", lo, hi)
+    assert eng.primes(lo, hi) == oracle.primes(lo, hi)
+
+
+def test_quotients_vs_pow(eng):
+    rng = random.Random(0x243F6A8885A308D3)
+    ps = [3, 5, 7, 9, 15, 1093, 3511, (1 << 50) - 27, (1 << 50) + 55, (1 << 63) - 25, (1 << 63) + 29,
+          18446744073709551557, (1 << 64) - 1]
+    for _ in range(4000):
+        bits = rng.randint(2, 64)
+        p = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
+        if p >= 3:
+            ps.append(p)
+    say("quotients", len(ps))
+    got = eng.quotients(ps)
+    M64 = (1 << 64) - 1
+    for p, (q, r) in zip(ps, got):
+        R = pow(2, p - 1, p * p)
+        assert r == R, p
+        assert q == ((R - 1) * pow(p, -1, 1 << 64)) & M64, p
+
+
+@pytest.mark.parametrize("w", range(len(WINS)))
+def test_golden_windows(eng, w):
+    g = WINS[w]
+    say("golden window", g["lo"], g["hi"])
+    r = eng.scan(g["lo"], g["hi"], near_T=g["near_T"])
+    assert r["primes"] == g["primes"]
+    assert "%016x" % r["cks"] == g["cks2"]
+    assert "%016x" % r["sumq"] == g["sumq2"]
+    assert r["flt"] == 0
+    assert near_of(r["events"]) == g["near"]
+    assert [p for t, p, q, rr in r["events"] if t == wf.EV_FOUND] == g["hits"]
+    assert r["tests"] == r["primes"] - (1 if g["lo"] <= 2 < g["hi"] else 0)
+
+
+@pytest.mark.parametrize("lo,hi", [(10 ** 15, 10 ** 15 + 2 * 10 ** 8), (10 ** 12 + 12345, 10 ** 12 + 10 ** 8 + 7),
+                                   ((1 << 50) - 10 ** 7, (1 << 50) + 10 ** 7), (3, 10 ** 8)])
+def test_scan_vs_oracle(eng, lo, hi):
+    say("scan vs oracle", lo, hi)
+    r = eng.scan(lo, hi, near_T=1000)
+    o = oracle.scan(lo, hi, bases=[2], near_T=1000)
+    assert (r["primes"], r["tests"], r["near"], r["flt"]) == (o["primes"], o["tests"], o["near"], o["flt"])
+    assert r["cks"] == o["cks"][2] and r["sumq"] == o["sumq"][2]
+    assert [(p, q) for t, p, q, rr in r["events"]] == [(e[2], e[3]) for e in o["events"]]
+
+
+def test_c1_pi_1e9(eng):
+    say("C1 [3, 1e9)")
+    r = eng.scan(3, 10 ** 9, near_T=100)
+    assert r["primes"] == FACTS["pi_1e9"] - 1
+    assert [p for t, p, q, rr in r["events"] if t == wf.EV_FOUND] == [1093, 3511]
+    o = oracle.scan(3, 10 ** 9, bases=[2], near_T=100)
+    assert r["cks"] == o["cks"][2] and r["sumq"] == o["sumq"][2] and r["near"] == o["near"]
+
+
+def test_batch_equals_single(eng):
+    rng = random.Random(7)
+    ranges = []
+    for _ in range(40):
+        lo = rng.randrange(10 ** 9, 10 ** 18)
+        ranges.append((lo, lo + rng.randrange(1, 3 * 10 ** 6)))
+    ranges += [(0, 100), (1 << 63, (1 << 63) + 10 ** 6), ((1 << 64) - 10 ** 6, 1 << 64), ((1 << 50) - 5, (1 << 50) + 5)]
+    say("batch", len(ranges))
+    res, ev = eng.scan_batch(ranges, near_T=0)
+    for (lo, hi), r in zip(ranges, res):
+        s = eng.scan(lo, hi)
+        assert (r["primes"], r["tests"], r["cks"], r["sumq"]) == (s["primes"], s["tests"], s["cks"], s["sumq"]), (lo, hi)
