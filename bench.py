@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Wieferich engine on the single-range workload.
+
+Workload (one step): every prime p in [1e15, 1e15 + 1e11) sieved on the GPU and
+Wieferich-tested for base 2 (q = (2^(p-1) mod p^2 - 1)/p, checksum, sum of q, near list
+|A| <= 100); all p are below 2^50, so the FP64 p-adic kernel runs.
+Inputs are generated on the device (the range itself is the input); nothing crosses
+PCIe in the timed region except the ~1 KB of totals and events per chunk.
+
+Prints ONE JSON line:  python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LO, SPAN = 10 ** 15, 10 ** 11
+NEAR_T = 100
+METRIC = "primes Wieferich-tested/sec at p~1e15, batched vs plain GPU, bit-exact cks"
+FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector peak (published), FMA = 2 flops
+FP64_PEAK_INSTR = 256 * 64 * 2.4e9   # wave64 FP64 VALU instructions: lanes per second
+INT32_MUL_PEAK = 256 * 32 * 2.4e9    # v_mad_u64_u32 products/s (half rate, measured)
+
+
+def wf_products(pbits):
+    """Algorithmic 32x32->64 partial products of one test (the survey's W(p))."""
+    L = math.ceil(2 * pbits / 32)
+    return (pbits - 1) * (L * (L + 1) // 2 + L + L * L) + (L + L * L)
+
+
+def fp64_flops(pbits, W=5):
+    """FP64 flops of one test in the p-adic kernel (FMA = 2): ceil(bits/W) windows of
+    W squarings (18 flops, 13 instructions) and one 2^d step (15 flops, 12 instructions)."""
+    nw = -(-pbits // W)
+    sq = (nw - 1) * W
+    flops = sq * 18 + nw * 15 + 12          # + set-up (1/p, 2/p, p/2) and normalisation
+    instr = sq * 13 + nw * 12 + 10
+    return flops, instr
+
+
+def cpu_baseline(target_s=8.0):
+    """The oracle (a C restatement of wieferich.c's algorithm, 8 lanes, window 5) on all
+    usable host cores, on a growing slice of the same workload until it runs >= target_s."""
+    import oracle
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    span = 10 ** 8
+    oracle.scan(LO, LO + 10 ** 6, bases=[2], threads=cores)       # sieving primes, warm caches
+    while True:
+        t = time.perf_counter()
+        r = oracle.scan(LO, LO + span, bases=[2], near_T=NEAR_T, threads=cores)
+        dt = time.perf_counter() - t
+        if dt >= target_s or span >= SPAN:
+            break
+        span = min(SPAN, span * max(2, int(target_s / max(dt, 1e-3) * 1.2)))
+    return {"value": r["tests"] / dt, "unit": "primes tested/s", "cores": cores, "kind": "port",
+            "sample": "[1e15, 1e15+%.3g) base 2, %d primes in %.2f s (oracle/wf_oracle.c)" % (span, r["primes"], dt)}
+
+
+def c3_ranges(n=4096, seed=0x243F6A8885A308D3):
+    """The batched workload: n ranges, starts uniform in [1e12, 1e18], lengths log-uniform in
+    [1e6, 1e9], from splitmix64 with a fixed seed (A001220/wieferich.c:1710-1718)."""
+    M = (1 << 64) - 1
+    st = seed
+
+    def nxt():
+        nonlocal st
+        st = (st + 0x9E3779B97F4A7C15) & M
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+    out = []
+    for _ in range(n):
+        u1 = nxt() / 2.0 ** 64
+        u2 = nxt() / 2.0 ** 64
+        lo = int(1e12 + u1 * (1e18 - 1e12))
+        ln = int(10 ** (6 + 3 * u2))
+        out.append((lo, lo + ln))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--arith", type=int, default=0, help="1 = force the 32-bit Montgomery kernels")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
+    ap.add_argument("--span", type=float, default=SPAN, help="numbers per step (default 1e11)")
+    ap.add_argument("--mode", choices=["both", "plain", "batched"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+              "(ranges shard by splitting [lo, hi), see DESIGN.md)", file=sys.stderr)
+        return 2
+    from mathpub_amd import wf
+    span = int(args.span)
+    eng = wf.Engine(arith=args.arith)
+    print("engine: " + eng.info(), file=sys.stderr, flush=True)
+
+    def timed(fn, label):
+        ref = None
+        for i in range(args.warmup):
+            t = time.perf_counter()
+            ref = fn()
+            print("%s warmup %d: %.3f s, %d primes, cks %016x" % (label, i, time.perf_counter() - t, ref["primes"],
+                                                                  ref["cks"]), file=sys.stderr, flush=True)
+        res, tim = [], []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            res.append(fn())
+            tim.append(eng.last_timing())
+        el = time.perf_counter() - t0
+        print("%s: %.3f s per step" % (label, el / args.steps), file=sys.stderr, flush=True)
+        return res + ([ref] if ref else []), tim, el / args.steps * 1e3
+
+    plain = batched = None
+    if args.mode in ("both", "plain"):
+        plain = timed(lambda: eng.scan(LO, LO + span, near_T=NEAR_T), "plain")
+    tuned = None
+    if args.mode in ("both", "batched"):
+        if not args.no_tune:
+            t = time.perf_counter()
+            tuned = eng.autotune(LO, LO + span)
+            print("autotune: %s (%.1f s)" % (tuned, time.perf_counter() - t), file=sys.stderr, flush=True)
+
+        def one():
+            per, ev = eng.scan_batch([(LO, LO + span)], near_T=NEAR_T)
+            return per[0]
+        batched = timed(one, "batched")
+    main_run = batched or plain
+    results, timings, ms_per_step = main_run
+    r = results[0]
+    same = all((x["primes"], x["cks"], x["sumq"], x["near"]) == (r["primes"], r["cks"], r["sumq"], r["near"])
+               for x in results + (plain[0] if plain else []))
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "c2_full.json")
+    if span == SPAN and os.path.exists(gpath):
+        g = json.load(open(gpath))
+        golden = (g["primes"] == r["primes"] and g["cks2"] == "%016x" % r["cks"]
+                  and g["sumq2"] == "%016x" % r["sumq"] and g["near"] == r["near"])
+    tests = r["tests"]
+    value = tests / (ms_per_step / 1e3)
+    roof = None
+    if plain:
+        # dominant kernel: the Fermat-quotient test, timed with HIP events on the engine's
+        # stream in the plain path (one sieve + one test launch per pass)
+        ptim = plain[1]
+        test_ms = sum(t["test_ms"] for t in ptim) / args.steps
+        sieve_ms = sum(t["sieve_ms"] for t in ptim) / args.steps
+        launches = max(1, ptim[-1]["launches"])
+        pbits = (LO + span // 2).bit_length()
+        flops, instr = fp64_flops(pbits)
+        if args.arith == 1 or (LO + span) > (1 << 50):
+            achieved = tests * wf_products(pbits) / (test_ms / 1e3) / 1e12
+            roof = {"bound": "valu", "achieved": achieved, "peak": INT32_MUL_PEAK / 1e12, "unit": "Tmul32/s",
+                    "frac": achieved / (INT32_MUL_PEAK / 1e12), "traffic": None, "kernel": "k_test<A_M4>"}
+        else:
+            achieved = tests * flops / (test_ms / 1e3) / 1e12
+            roof = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_test<A_F64>",
+                    "valu_fp64_issue_frac": tests * instr / (test_ms / 1e3) / FP64_PEAK_INSTR}
+        roof["avg_launch_ms"] = test_ms / launches
+        roof["launches_per_step"] = launches
+        roof["w_products_per_s"] = tests * wf_products(pbits) / (test_ms / 1e3)
+        roof["sieve_ms_per_step"] = sieve_ms
+        roof["test_ms_per_step"] = test_ms
+    cfg = {"workload": "single range [1e15, 1e15+%.3g), base 2, near |A|<=%d" % (span, NEAR_T),
+           "engine": "batched" if batched else "plain", "primes_per_step": r["primes"], "tests_per_step": tests,
+           "cks": "%016x" % r["cks"], "sumq": "%016x" % r["sumq"], "near": r["near"],
+           "bit_exact_across_steps_and_engines": same, "bit_exact_vs_golden": golden, "parallelism": "1 GPU",
+           "autotuned": tuned}
+    if plain and batched:
+        pv = tests / (plain[2] / 1e3)
+        cfg["plain_value"] = pv
+        cfg["plain_ms_per_step"] = plain[2]
+        cfg["batched_over_plain"] = value / pv
+    out = {
+        "metric": METRIC, "value": value, "unit": "primes tested/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if (roof or {}).get("unit") == "TFLOP/s" else "u32",
+        "data": "synthetic (the range is the input; primes sieved on the GPU)",
+        "config": cfg, "roofline": roof, "cpu_baseline": None,
+    }
+    eng.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

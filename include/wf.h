@@ -37,6 +37,9 @@ extern "C" {
 
 #define WF_MAXB 64
 
+/* Opaque engine: one per GPU, owned by one host thread. */
+typedef struct wf_engine wf_engine;
+
 enum {
     WF_OK = 0,
     WF_E_ARG = -1,          /* bad argument (range, sizes, NULL pointer) */
@@ -74,52 +77,59 @@ typedef struct {
     uint32_t sieve_block;   /* threads per sieve workgroup (256 .. 1024) */
     uint32_t test_block;    /* threads per test workgroup (64 .. 1024) */
     uint32_t arith;         /* 0 = fastest exact path, 1 = force 32-bit Montgomery */
+    uint32_t bucket_min;    /* sieving primes >= this go through bucket records (>= 512) */
     uint64_t chunk;         /* numbers per device pass of wf_scan (memory bound) */
 } wf_config;
 
 /* Engine life cycle (the reference's gpu_init, wieferich_cuda.cu:1570-1577). */
-int  wf_open(int device, const wf_config *cfg, struct wf_engine **out);
-void wf_close(struct wf_engine *e);
+int  wf_open(int device, const wf_config *cfg, wf_engine **out);
+void wf_close(wf_engine *e);
 const char *wf_strerror(int rc);
 /* Device name, CU count and the arithmetic path, for status lines. */
-int  wf_info(struct wf_engine *e, char *buf, int len);
+int  wf_info(wf_engine *e, char *buf, int len);
 
 /* Bases of the run, ascending and distinct.  The GPU kernels implement base 2. */
-int  wf_set_bases(struct wf_engine *e, const uint32_t *bases, int nb);
+int  wf_set_bases(wf_engine *e, const uint32_t *bases, int nb);
 
 /* Scan one range: totals into *out, events sorted by (p, j) into ev[0..ev_cap).
  * near_T > 0 reports 0 < |A| <= near_T (wieferich.c:1065-1073).  want_psp != 0 also
  * lists the odd composites of the range that are base-2 strong probable primes, sorted,
  * into psp[0..psp_cap) (out->npsp counts all of them). */
-int  wf_scan(struct wf_engine *e, uint64_t lo, uint64_t hi, int64_t near_T, wf_result *out,
+int  wf_scan(wf_engine *e, uint64_t lo, uint64_t hi, int64_t near_T, wf_result *out,
              wf_event *ev, uint64_t ev_cap, int want_psp, uint64_t *psp, uint64_t psp_cap);
 
 /* Scan n ranges in one device pass: per_range[i] receives range i's totals; events of
  * all ranges go to ev (sorted by (p, j); a range is found by p). */
-int  wf_scan_batch(struct wf_engine *e, const wf_range *r, uint32_t n, int64_t near_T,
+int  wf_scan_batch(wf_engine *e, const wf_range *r, uint32_t n, int64_t near_T,
                    wf_result *per_range, wf_event *ev, uint64_t ev_cap);
 
 /* Fermat quotients of arbitrary odd p (prime or not) for base 2: r = 2^(p-1) mod p^2,
  * q = (r - 1) p^-1 mod 2^64, as fermat_quotient() (wieferich.c:939-949). */
-int  wf_quotients(struct wf_engine *e, const uint64_t *p, uint64_t *q, uint64_t *r_hi,
+int  wf_quotients(wf_engine *e, const uint64_t *p, uint64_t *q, uint64_t *r_hi,
                   uint64_t *r_lo, uint64_t n);
 
 /* The primes of [lo, hi) in ascending order into out[0..cap); *count = how many exist. */
-int  wf_primes(struct wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *out, uint64_t cap,
+int  wf_primes(wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *out, uint64_t cap,
                uint64_t *count);
 
 /* Serving: a device-resident request loop with continuous batching.  submit admits a
  * range when a slot and the HBM budget allow (else WF_E_BUSY); poll returns its result
  * once done (*done = 1).  Results equal wf_scan of the same range. */
-int  wf_serve_start(struct wf_engine *e, uint32_t slots, uint64_t hbm_budget);
-int  wf_serve_submit(struct wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *ticket);
-int  wf_serve_poll(struct wf_engine *e, uint64_t ticket, wf_result *out, int *done);
-int  wf_serve_stop(struct wf_engine *e);
+int  wf_serve_start(wf_engine *e, uint32_t slots, uint64_t hbm_budget);
+int  wf_serve_submit(wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *ticket);
+int  wf_serve_poll(wf_engine *e, uint64_t ticket, wf_result *out, int *done);
+int  wf_serve_stop(wf_engine *e);
+
+/* Autotuner: times the sieve stage on a sample of [lo, hi) over segment sizes 16-64 KiB,
+ * sieve workgroups of 4-16 wavefronts and bucket thresholds 2^15-2^18, switches the engine
+ * to the fastest and returns it (*best_ms = its sieve milliseconds per 1e9 numbers).
+ * No counterpart in the reference (its chunk and thread counts are set by hand). */
+int  wf_autotune(wf_engine *e, uint64_t lo, uint64_t hi, wf_config *best, double *best_ms);
 
 /* Timings of the last wf_scan / wf_scan_batch: device milliseconds of the sieve and
  * the test kernels (HIP events on the engine's stream), primes tested. */
 typedef struct { double sieve_ms, test_ms, total_ms; uint64_t tested; uint64_t launches; } wf_timing;
-int  wf_last_timing(struct wf_engine *e, wf_timing *t);
+int  wf_last_timing(wf_engine *e, wf_timing *t);
 
 #ifdef __cplusplus
 }

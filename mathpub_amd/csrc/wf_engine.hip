@@ -90,6 +90,16 @@ __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c) {
 // ---------------------------------------------------------------------------------
 // sieve
 // ---------------------------------------------------------------------------------
+//
+// A pass covers a list of segments.  Segment s holds the wheel-30 bitmap of
+// [base, base + 30 nbytes) in LDS.  Sieving primes below the bucket threshold Q1 are
+// applied directly per segment (tier A: q < 512, one wavefront per prime; tier B: one
+// lane per prime).  Primes >= Q1 hit a segment at most a few times; k_bucket_fill walks
+// each such prime over a run of consecutive segments once and leaves one 4-byte record
+// (byte << 3 | class) per hit in the bucket of the segment it lands in.
+
+#define RUN_MAX 4096      // segments per run (LDS cursors of k_bucket_fill)
+#define BF_PRIMES 2048    // large primes per k_bucket_fill workgroup
 
 struct SegDesc {
   u64 base;       // multiple of 30
@@ -100,6 +110,9 @@ struct SegDesc {
   u32 nq;         // sieving primes with q^2 <= the segment's last number
   u32 pad;
 };
+
+// consecutive segments seg0 .. seg0 + nseg - 1 of one range piece, base = seg0's base
+struct RunDesc { u64 base; u32 seg0, nseg, nq, pad; };
 
 struct DevAcc {     // per range; every field is a 64-bit sum
   unsigned long long primes, tests, near, flt, cks, sumq, nev, npsp;
@@ -120,13 +133,78 @@ __device__ __forceinline__ bool first_offset(u64 base, u32 q, u64 minv, u64 span
   return o < span;
 }
 
+// Large primes sp[q1idx .. run.nq) -> bucket records of the run's segments.  Two walks
+// over the hits: the first counts per segment in LDS and reserves room in each bucket
+// with one global atomic per (workgroup, segment), the second writes the records.
+__global__ void __launch_bounds__(256) k_bucket_fill(const RunDesc* __restrict__ runs, u32 npb,
+                                                     const u32* __restrict__ sp, const u64* __restrict__ spinv,
+                                                     u32 q1idx, u32* __restrict__ bcnt, u32* __restrict__ bdata,
+                                                     u32 bcap, int sbshift, u32* __restrict__ flags) {
+  __shared__ u32 cur[RUN_MAX];
+  const RunDesc r = runs[blockIdx.x / npb];
+  const u32 i0 = q1idx + (blockIdx.x % npb) * BF_PRIMES;
+  if (i0 >= r.nq) return;
+  const u32 i1 = min(i0 + BF_PRIMES, r.nq);
+  const u64 nbytes = (u64)r.nseg << sbshift;
+  const u64 span = 30ull * nbytes;
+  const u32 smask = (1u << sbshift) - 1;
+  for (u32 k = threadIdx.x; k < r.nseg; k += blockDim.x) cur[k] = 0;
+  __syncthreads();
+  for (u32 i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const u32 q = sp[i];
+    u64 o;
+    if (!first_offset(r.base, q, spinv[i], span, o)) continue;
+    const int qi = c_QI[q % 30];
+    for (int c = 0; c < 8; c++)
+      for (u64 j = class_start(o, q, qi, c); j < nbytes; j += q) atomicAdd(&cur[j >> sbshift], 1u);
+  }
+  __syncthreads();
+  for (u32 k = threadIdx.x; k < r.nseg; k += blockDim.x) {
+    const u32 h = cur[k];
+    if (h) {
+      const u32 b = atomicAdd(&bcnt[r.seg0 + k], h);
+      if (b + h > bcap) atomicOr(flags, 1u);
+      cur[k] = b;
+    }
+  }
+  __syncthreads();
+  for (u32 i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const u32 q = sp[i];
+    u64 o;
+    if (!first_offset(r.base, q, spinv[i], span, o)) continue;
+    const int qi = c_QI[q % 30];
+    for (int c = 0; c < 8; c++)
+      for (u64 j = class_start(o, q, qi, c); j < nbytes; j += q) {
+        const u32 s = (u32)(j >> sbshift);
+        const u32 slot = atomicAdd(&cur[s], 1u);
+        if (slot < bcap) bdata[(u64)(r.seg0 + s) * bcap + slot] = (((u32)j & smask) << 3) | (u32)c;
+      }
+  }
+}
+
+// Valid-bit mask of LDS word w (bytes 4w .. 4w+3) for numbers base + [lo_off, hi_off).
+__device__ __forceinline__ u32 word_mask(u32 w, u32 lo_off, u32 hi_off) {
+  const u32 w0 = 120u * w;
+  if (w0 >= lo_off && w0 + 120u <= hi_off) return 0xFFFFFFFFu;
+  u32 m = 0;
+  for (int b = 0; b < 32; b++) {
+    const u32 off = w0 + 30u * (b >> 3) + c_R30[b & 7];
+    if (off >= lo_off && off < hi_off) m |= 1u << b;
+  }
+  return m;
+}
+
+// Sieve one segment per workgroup iteration and write its primes, in ascending order,
+// as offsets from the segment base into out[s * pcap ...]; cnt[s] = how many.
 __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs, u32 nseg,
                                                 const u32* __restrict__ sp, const u64* __restrict__ spinv,
-                                                u32 tierA,
-                                                u64* __restrict__ out_p, u32* __restrict__ out_rid,
-                                                unsigned long long* __restrict__ ctr, u64 cap,
-                                                DevAcc* __restrict__ acc) {
+                                                u32 tierA, u32 q1idx,
+                                                const u32* __restrict__ bcnt, const u32* __restrict__ bdata,
+                                                u32 bcap,
+                                                u32* __restrict__ out, u32* __restrict__ cnt, u32 pcap,
+                                                DevAcc* __restrict__ acc, u32* __restrict__ flags) {
   extern __shared__ u32 seg[];
+  __shared__ u32 wtot[16];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
@@ -136,8 +214,9 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
     const u64 span = 30ull * d.nbytes;
     for (u32 w = threadIdx.x; w < nwords; w += blockDim.x) seg[w] = 0;
     __syncthreads();
+    const u32 nsmall = min(d.nq, q1idx);
     // tier A: one wavefront per prime, lanes stride over the marks of each class
-    for (u32 i = wave; i < min(tierA, d.nq); i += nwaves) {
+    for (u32 i = wave; i < min(tierA, nsmall); i += nwaves) {
       const u32 q = sp[i];
       u64 o;
       if (!first_offset(d.base, q, spinv[i], span, o)) continue;
@@ -149,7 +228,7 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
       }
     }
     // tier B: one lane per prime
-    for (u32 i = tierA + threadIdx.x; i < d.nq; i += blockDim.x) {
+    for (u32 i = tierA + threadIdx.x; i < nsmall; i += blockDim.x) {
       const u32 q = sp[i];
       u64 o;
       if (!first_offset(d.base, q, spinv[i], span, o)) continue;
@@ -160,60 +239,63 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
           atomicOr(&seg[j >> 2], 1u << ((((u32)j & 3) << 3) | c));
       }
     }
-    __syncthreads();
-    // extraction: survivors of each word within [lo_off, hi_off)
-    u32 cnt = 0;
-    for (u32 w = threadIdx.x; w < nwords; w += blockDim.x) {
-      u32 v = ~seg[w];
-      const u32 w0 = 120u * w;  // offset of the word's first byte
-      if (w0 < d.lo_off || w0 + 120u > d.hi_off) {
-        u32 m = 0;
-        for (int b = 0; b < 32; b++) {
-          const u32 off = w0 + 30u * (b >> 3) + c_R30[b & 7];
-          if (off >= d.lo_off && off < d.hi_off) m |= 1u << b;
-        }
-        v &= m;
+    // large primes: this segment's bucket
+    if (d.nq > q1idx) {
+      const u32 nrec = min(bcnt[s], bcap);
+      const u32* rec = bdata + (u64)s * bcap;
+      for (u32 k = threadIdx.x; k < nrec; k += blockDim.x) {
+        const u32 v = rec[k];
+        atomicOr(&seg[v >> 5], 1u << ((((v >> 3) & 3) << 3) | (v & 7)));
       }
-      cnt += __popc(v);
     }
-    // the primes 3 and 5 are not on the wheel
+    __syncthreads();
+    // extraction: wave w owns words [w M, (w + 1) M); lanes read consecutive words
+    const u32 M = (nwords + nwaves - 1) / nwaves;
+    const u32 wb = min((u32)wave * M, nwords), we = min(wb + M, nwords);
+    u32 c0 = 0;
+    for (u32 w = wb + lane; w < we; w += 64) c0 += __popc(~seg[w] & word_mask(w, d.lo_off, d.hi_off));
+    // the primes 3 and 5 are not on the wheel; they lead segment 0
     u32 extra = 0;
-    if (d.base == 0 && threadIdx.x == 0) {
+    if (d.base == 0) {
       if (3 >= d.lo_off && 3 < d.hi_off) extra |= 1;
       if (5 >= d.lo_off && 5 < d.hi_off) extra |= 2;
-      cnt += __popc(extra);
     }
-    const u32 incl = wave_incl_scan(cnt);
-    const u32 tot = __shfl(incl, 63, 64);
-    unsigned long long wbase = 0;
-    if (lane == 63 && tot) {
-      wbase = atomicAdd(&ctr[0], (unsigned long long)tot);
-      atomicAdd(&acc[d.rid].primes, (unsigned long long)tot);
+    const u32 wsum = (u32)wave_sum((u64)c0) + (wave == 0 ? __popc(extra) : 0);
+    if (lane == 0) wtot[wave] = wsum;
+    __syncthreads();
+    u32 pos = 0, total = 0;
+    for (int k = 0; k < nwaves; k++) {
+      const u32 t = wtot[k];
+      if (k < wave) pos += t;
+      total += t;
     }
-    wbase = __shfl(wbase, 63, 64);
-    u64 pos = wbase + incl - cnt;
-    if (extra & 1) { if (pos < cap) { out_p[pos] = 3; out_rid[pos] = d.rid; } pos++; }
-    if (extra & 2) { if (pos < cap) { out_p[pos] = 5; out_rid[pos] = d.rid; } pos++; }
-    for (u32 w = threadIdx.x; w < nwords; w += blockDim.x) {
-      u32 v = ~seg[w];
-      const u32 w0 = 120u * w;
-      if (w0 < d.lo_off || w0 + 120u > d.hi_off) {
-        u32 m = 0;
-        for (int b = 0; b < 32; b++) {
-          const u32 off = w0 + 30u * (b >> 3) + c_R30[b & 7];
-          if (off >= d.lo_off && off < d.hi_off) m |= 1u << b;
-        }
-        v &= m;
+    u32* o = out + (u64)s * pcap;
+    if (wave == 0) {
+      if (lane == 0) {
+        u32 k = 0;
+        if ((extra & 1) && k < pcap) o[k++] = 3;
+        if ((extra & 2) && k < pcap) o[k++] = 5;
       }
+      pos += __popc(extra);
+    }
+    for (u32 w0 = wb; w0 < we; w0 += 64) {
+      const u32 w = w0 + lane;
+      u32 v = w < we ? (~seg[w] & word_mask(w, d.lo_off, d.hi_off)) : 0;
+      const u32 c = __popc(v);
+      const u32 incl = wave_incl_scan(c);
+      u32 k = pos + incl - c;
       while (v) {
         const int b = __ffs(v) - 1;
         v &= v - 1;
-        if (pos < cap) {
-          out_p[pos] = d.base + w0 + 30u * (b >> 3) + c_R30[b & 7];
-          out_rid[pos] = d.rid;
-        }
-        pos++;
+        if (k < pcap) o[k] = 120u * w + 30u * (b >> 3) + c_R30[b & 7];
+        k++;
       }
+      pos += __shfl(incl, 63, 64);
+    }
+    if (threadIdx.x == 0) {
+      cnt[s] = min(total, pcap);
+      if (total > pcap) atomicOr(flags, 2u);
+      if (total) atomicAdd(&acc[d.rid].primes, (unsigned long long)total);
     }
     __syncthreads();
   }
@@ -222,6 +304,8 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
 // ---------------------------------------------------------------------------------
 // tests
 // ---------------------------------------------------------------------------------
+
+#define TEST_TILE 1024   // primes per test work item (4 rounds of 256 lanes)
 
 struct TestOut {
   DevAcc* acc;
@@ -242,7 +326,7 @@ __device__ __forceinline__ void push_event(const TestOut& o, u32 type, u32 rid, 
 }
 
 // Account one tested prime exactly as finish_lanes (wieferich.c:1051-1075) and add the
-// lane's terms to the wavefront sums.  r = r_hi:r_lo, q = (r - 1) p^-1 mod 2^64.
+// lane's terms to its running sums.  r = r_hi:r_lo, q = (r - 1) p^-1 mod 2^64.
 struct LaneSums { u64 tests, flt, near, cks, sumq; };
 
 __device__ __forceinline__ void account(const TestOut& o, u32 rid, u64 p, u64 rhi, u64 rlo, LaneSums& s) {
@@ -267,31 +351,19 @@ __device__ __forceinline__ void account(const TestOut& o, u32 rid, u64 p, u64 rh
   }
 }
 
-// Wavefront reduction of the lane sums into the range's accumulator; lanes of one
-// wavefront usually share a range, otherwise each lane adds its own terms.
-__device__ __forceinline__ void flush_sums(const TestOut& o, u32 rid, bool active, const LaneSums& s) {
-  const int lane = threadIdx.x & 63;
-  const u32 r0 = __shfl(rid, 0, 64);
-  const bool uniform = __all(!active || rid == r0);
-  if (uniform) {
-    const u64 t = wave_sum(s.tests), f = wave_sum(s.flt), n = wave_sum(s.near);
-    const u64 c = wave_sum(s.cks), q = wave_sum(s.sumq);
-    if (lane == 0) {
-      DevAcc* a = &o.acc[r0];
-      if (t) atomicAdd(&a->tests, t);
-      if (f) atomicAdd(&a->flt, f);
-      if (n) atomicAdd(&a->near, n);
-      atomicAdd(&a->cks, c);
-      atomicAdd(&a->sumq, q);
-    }
-  } else if (active) {
+// Wavefront sums of a range's lane totals -> one atomic per counter (lane 0).
+__device__ __forceinline__ void flush_sums(const TestOut& o, u32 rid, LaneSums& s) {
+  const u64 t = wave_sum(s.tests), f = wave_sum(s.flt), n = wave_sum(s.near);
+  const u64 c = wave_sum(s.cks), q = wave_sum(s.sumq);
+  if ((threadIdx.x & 63) == 0 && t) {
     DevAcc* a = &o.acc[rid];
-    atomicAdd(&a->tests, s.tests);
-    if (s.flt) atomicAdd(&a->flt, s.flt);
-    if (s.near) atomicAdd(&a->near, s.near);
-    atomicAdd(&a->cks, s.cks);
-    atomicAdd(&a->sumq, s.sumq);
+    atomicAdd(&a->tests, t);
+    if (f) atomicAdd(&a->flt, f);
+    if (n) atomicAdd(&a->near, n);
+    atomicAdd(&a->cks, c);
+    atomicAdd(&a->sumq, q);
   }
+  s.tests = s.flt = s.near = s.cks = s.sumq = 0;
 }
 
 // 128-bit r = a + b p from the normalised p-adic digits
@@ -303,54 +375,58 @@ __device__ __forceinline__ void padic_to_r(u64 p, double a, double b, u64& rhi, 
   rhi = hi + (rlo < lo);
 }
 
-template <int W>
-__global__ void __launch_bounds__(256) k_test_f64(const u64* __restrict__ primes, const u32* __restrict__ rid,
-                                                  const unsigned long long* __restrict__ count, u64 cap,
-                                                  TestOut o) {
-  const u64 n = min((u64)*count, cap);   // the sieve never writes past cap
-  const u64 stride = (u64)gridDim.x * blockDim.x;
-  for (u64 base = (u64)blockIdx.x * blockDim.x; base < n; base += stride) {
-    const u64 i = base + threadIdx.x;
-    const bool active = i < n;
-    const u64 p = active ? primes[i] : 3;
-    const u32 r = active ? rid[i] : 0;
-    const u64 e = p - 1;
-    const int eb = wave_max(64 - __builtin_clzll(e));
+// One prime per lane over the segments' prime lists; workgroups take segments in turn
+// and keep their sums until the range changes (uniform across the workgroup).
+template <int ARITH>
+__device__ __forceinline__ void residue(u64 p, int eb, u64& rhi, u64& rlo) {
+  const u64 e = p - 1;
+  if (ARITH == 0) {
     double a, b;
-    fp64_pow2<W>(p, e, eb, a, b);
-    LaneSums s = {0, 0, 0, 0, 0};
-    if (active) {
-      u64 rhi, rlo;
-      padic_to_r(p, a, b, rhi, rlo);
-      account(o, r, p, rhi, rlo, s);
-    }
-    flush_sums(o, r, active, s);
+    fp64_pow2<5>(p, e, eb, a, b);
+    padic_to_r(p, a, b, rhi, rlo);
+  } else {
+    constexpr int L = ARITH == 1 ? 2 : (ARITH == 2 ? 3 : 4);
+    constexpr bool FULL = ARITH == 4;
+    u32 x[L];
+    mont_pow2<L, FULL>(p, e, eb, x);
+    rlo = (u64)x[0] | ((u64)x[1] << 32);
+    rhi = (L > 2 ? (u64)x[2] : 0) | (L > 3 ? ((u64)x[3] << 32) : 0);
   }
 }
 
-template <int L, bool FULL>
-__global__ void __launch_bounds__(256) k_test_mont(const u64* __restrict__ primes, const u32* __restrict__ rid,
-                                                   const unsigned long long* __restrict__ count, u64 cap,
-                                                   TestOut o) {
-  const u64 n = min((u64)*count, cap);
-  const u64 stride = (u64)gridDim.x * blockDim.x;
-  for (u64 base = (u64)blockIdx.x * blockDim.x; base < n; base += stride) {
-    const u64 i = base + threadIdx.x;
-    const bool active = i < n;
-    const u64 p = active ? primes[i] : (FULL ? 0xFFFFFFFFFFFFFFC5ull : (L == 2 ? 3ull : (L == 3 ? 2147483659ull : 140737488355333ull)));
-    const u32 r = active ? rid[i] : 0;
-    const u64 e = p - 1;
-    const int eb = wave_max(64 - __builtin_clzll(e));
-    u32 x[L];
-    mont_pow2<L, FULL>(p, e, eb, x);
-    LaneSums s = {0, 0, 0, 0, 0};
-    if (active) {
-      const u64 rlo = (u64)x[0] | ((u64)x[1] << 32);
-      const u64 rhi = (L > 2 ? (u64)x[2] : 0) | (L > 3 ? ((u64)x[3] << 32) : 0);
-      account(o, r, p, rhi, rlo, s);
+template <int ARITH>
+__global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, u32 nseg,
+                                              const u32* __restrict__ off, const u32* __restrict__ cnt, u32 pcap,
+                                              u32 tps, TestOut o) {
+  LaneSums s = {0, 0, 0, 0, 0};
+  u32 cur = 0xFFFFFFFFu;
+  // a valid modulus of the class for idle lanes
+  const u64 idle = ARITH == 0 ? 3ull : ARITH == 1 ? 3ull : ARITH == 2 ? 2147483659ull
+                 : ARITH == 3 ? 140737488355333ull : 0xFFFFFFFFFFFFFFC5ull;
+  const u64 ntiles = (u64)nseg * tps;
+  for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const u32 sg = (u32)(t / tps);
+    const u32 beg = (u32)(t - (u64)sg * tps) * TEST_TILE;
+    const u32 n = cnt[sg];
+    if (beg >= n) continue;                       // uniform across the workgroup
+    const u32 end = min(n, beg + TEST_TILE);
+    const SegDesc d = segs[sg];
+    if (d.rid != cur) {
+      if (cur != 0xFFFFFFFFu) flush_sums(o, cur, s);
+      cur = d.rid;
     }
-    flush_sums(o, r, active, s);
+    const u32* po = off + (u64)sg * pcap;
+    for (u32 base = beg; base < end; base += blockDim.x) {
+      const u32 i = base + threadIdx.x;
+      const bool active = i < end;
+      const u64 p = active ? d.base + po[i] : idle;
+      const int eb = wave_max(64 - __builtin_clzll(p - 1));
+      u64 rhi, rlo;
+      residue<ARITH>(p, eb, rhi, rlo);
+      if (active) account(o, d.rid, p, rhi, rlo, s);
+    }
   }
+  if (cur != 0xFFFFFFFFu) flush_sums(o, cur, s);
 }
 
 // Quotients of arbitrary odd p (check / selftest): per-lane choice of the path.
@@ -359,27 +435,19 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const u64 p = p_in[i];
-  const u64 e = p - 1;
-  const int eb = 64 - __builtin_clzll(e);
+  const int eb = 64 - __builtin_clzll(p - 1);
   u64 rhi, rlo;
-  if (p < (1ull << 50)) {
-    double a, b;
-    fp64_pow2<5>(p, e, eb, a, b);
-    padic_to_r(p, a, b, rhi, rlo);
-  } else if (p < (1ull << 63)) {
-    u32 x[4];
-    mont_pow2<4, false>(p, e, eb, x);
-    rlo = (u64)x[0] | ((u64)x[1] << 32);
-    rhi = (u64)x[2] | ((u64)x[3] << 32);
-  } else {
-    u32 x[4];
-    mont_pow2<4, true>(p, e, eb, x);
-    rlo = (u64)x[0] | ((u64)x[1] << 32);
-    rhi = (u64)x[2] | ((u64)x[3] << 32);
-  }
+  if (p < (1ull << 50)) residue<0>(p, eb, rhi, rlo);
+  else if (p < (1ull << 63)) residue<3>(p, eb, rhi, rlo);
+  else residue<4>(p, eb, rhi, rlo);
   q_out[i] = (rlo - 1) * inv64(p);
   rhi_out[i] = rhi;
   rlo_out[i] = rlo;
+}
+
+// Zero n words (a kernel, so that the batched path can record it into a HIP graph).
+__global__ void k_zero(u32* __restrict__ p, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -388,31 +456,47 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
 
 static const u32 R30H[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 
+struct BufSet {   // bucket counters / records, prime offsets / counts of one pass
+  u32* bcnt = nullptr;
+  u64 bcnt_cap = 0;
+  u32* bdata = nullptr;
+  u64 bdata_cap = 0;
+  u32* off = nullptr;
+  u64 off_cap = 0;
+  u32* cnt = nullptr;
+  u64 cnt_cap = 0;
+};
+
 struct wf_engine {
   int dev = 0;
   hipStream_t st = nullptr;
   hipDeviceProp_t prop;
   wf_config cfg;
+  int sbshift = 15;
   int nb = 1;
   u32 bases[WF_MAXB] = {2};
-  // sieving primes (odd primes >= 7, ascending)
+  // sieving primes (odd primes >= 7, ascending) and prefix sums of 1/q
   std::vector<u32> h_sp;
+  std::vector<double> h_inv_sum;
   u64 sp_lim = 0;
   u32* d_sp = nullptr;
   u64* d_spinv = nullptr;
-  u32 tierA = 0;
-  // work buffers
+  u32 tierA = 0, q1idx = 0;
+  // pass descriptors (all passes of a batch) and two sets of pass buffers
   SegDesc* d_segs = nullptr;
   u64 seg_cap = 0;
-  u64* d_primes = nullptr;
-  u32* d_rid = nullptr;
-  u64 prime_cap = 0, rid_cap = 0;
+  RunDesc* d_runs = nullptr;
+  u64 run_cap = 0;
+  BufSet bs[2];
+  u32* d_flags = nullptr;
+  hipStream_t st2 = nullptr;     // test kernels of the batched pipeline
+  hipEvent_t gev[5] = {};        // graph capture: fork, sieved[2], tested[2]
   DevAcc* d_acc = nullptr;
   u64 acc_cap = 0;
   DevEvent* d_ev = nullptr;
   u64 dev_ev_cap = 0;
-  unsigned long long* d_ctr = nullptr;   // [0] primes appended, [1] events
-  hipEvent_t ev[4];
+  unsigned long long* d_ctr = nullptr;   // [1] events
+  hipEvent_t ev[4] = {};
   wf_timing last;
 };
 
@@ -479,8 +563,8 @@ static int ensure_sieving_primes(wf_engine* e, u64 lim) {
   host_primes(want, e->h_sp);
   std::vector<u64> inv(e->h_sp.size());
   for (size_t i = 0; i < e->h_sp.size(); i++) inv[i] = 0xFFFFFFFFFFFFFFFFull / e->h_sp[i];
-  if (e->d_sp) hipFree(e->d_sp);
-  if (e->d_spinv) hipFree(e->d_spinv);
+  if (e->d_sp) (void)hipFree(e->d_sp);
+  if (e->d_spinv) (void)hipFree(e->d_spinv);
   e->d_sp = nullptr;
   e->d_spinv = nullptr;
   const size_t n = std::max<size_t>(e->h_sp.size(), 1);
@@ -491,13 +575,17 @@ static int ensure_sieving_primes(wf_engine* e, u64 lim) {
   e->sp_lim = want;
   // tier A: primes below 64 * 8 = 512 are striped over a wavefront
   e->tierA = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), 512u) - e->h_sp.begin());
+  // primes >= the bucket threshold go through k_bucket_fill
+  e->q1idx = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), e->cfg.bucket_min) - e->h_sp.begin());
+  e->h_inv_sum.assign(e->h_sp.size() + 1, 0.0);
+  for (size_t i = 0; i < e->h_sp.size(); i++) e->h_inv_sum[i + 1] = e->h_inv_sum[i] + 1.0 / e->h_sp[i];
   return WF_OK;
 }
 
 template <typename T>
 static int grow(T*& ptr, u64& cap, u64 need) {
   if (need <= cap) return WF_OK;
-  if (ptr) hipFree(ptr);
+  if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   u64 n = std::max<u64>(need, cap + cap / 2);
   if (hipMalloc(&ptr, n * sizeof(T)) != hipSuccess) {
@@ -523,14 +611,25 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
   if (!e->cfg.sieve_block) e->cfg.sieve_block = 512;
   if (!e->cfg.test_block) e->cfg.test_block = 256;
   if (!e->cfg.chunk) e->cfg.chunk = 10000000000ull;
+  if (!e->cfg.bucket_min) e->cfg.bucket_min = 65536;
   if (e->cfg.seg_bytes < 4096 || (e->cfg.seg_bytes & (e->cfg.seg_bytes - 1)) ||
-      e->cfg.seg_bytes > e->prop.sharedMemPerBlock || e->cfg.sieve_block < 64 || e->cfg.sieve_block > 1024 ||
-      (e->cfg.sieve_block & 63) || e->cfg.test_block != 256) {
+      e->cfg.seg_bytes > 65536 || e->cfg.seg_bytes + 64 > e->prop.sharedMemPerBlock || e->cfg.sieve_block < 64 ||
+      e->cfg.sieve_block > 1024 || (e->cfg.sieve_block & 63) || e->cfg.test_block != 256 || e->cfg.arith > 1 ||
+      e->cfg.bucket_min < 512 || e->cfg.chunk < 30ull * e->cfg.seg_bytes) {
     delete e;
     return WF_E_ARG;
   }
-  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) { delete e; return WF_E_DEVICE; }
-  for (int i = 0; i < 4; i++) hipEventCreate(&e->ev[i]);
+  e->sbshift = __builtin_ctz(e->cfg.seg_bytes);
+  // both streams at the default priority
+  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking) != hipSuccess) {
+    wf_close(e);
+    return WF_E_DEVICE;
+  }
+  for (int i = 0; i < 4; i++)
+    if (hipEventCreate(&e->ev[i]) != hipSuccess) { wf_close(e); return WF_E_DEVICE; }
+  for (int i = 0; i < 5; i++)
+    if (hipEventCreateWithFlags(&e->gev[i], hipEventDisableTiming) != hipSuccess) { wf_close(e); return WF_E_DEVICE; }
   // wheel tables
   unsigned char m30[8 * 30 * 8], qi[30];
   memset(qi, 0, sizeof qi);
@@ -549,7 +648,11 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
     wf_close(e);
     return WF_E_DEVICE;
   }
-  if (hipMalloc(&e->d_ctr, 8 * sizeof(unsigned long long)) != hipSuccess) { wf_close(e); return WF_E_NOMEM; }
+  if (hipMalloc(&e->d_ctr, 8 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&e->d_flags, 8 * sizeof(u32)) != hipSuccess) {
+    wf_close(e);
+    return WF_E_NOMEM;
+  }
   memset(&e->last, 0, sizeof e->last);
   *out = e;
   return WF_OK;
@@ -557,18 +660,28 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
 
 extern "C" void wf_close(wf_engine* e) {
   if (!e) return;
-  hipSetDevice(e->dev);
-  if (e->st) hipStreamSynchronize(e->st);
-  hipFree(e->d_sp);
-  hipFree(e->d_spinv);
-  hipFree(e->d_segs);
-  hipFree(e->d_primes);
-  hipFree(e->d_rid);
-  hipFree(e->d_acc);
-  hipFree(e->d_ev);
-  hipFree(e->d_ctr);
-  for (int i = 0; i < 4; i++) hipEventDestroy(e->ev[i]);
-  if (e->st) hipStreamDestroy(e->st);
+  (void)hipSetDevice(e->dev);
+  if (e->st) (void)hipStreamSynchronize(e->st);
+  (void)hipFree(e->d_sp);
+  (void)hipFree(e->d_spinv);
+  (void)hipFree(e->d_segs);
+  (void)hipFree(e->d_runs);
+  for (int k = 0; k < 2; k++) {
+    (void)hipFree(e->bs[k].bcnt);
+    (void)hipFree(e->bs[k].bdata);
+    (void)hipFree(e->bs[k].off);
+    (void)hipFree(e->bs[k].cnt);
+  }
+  (void)hipFree(e->d_flags);
+  (void)hipFree(e->d_acc);
+  (void)hipFree(e->d_ev);
+  (void)hipFree(e->d_ctr);
+  for (int i = 0; i < 4; i++)
+    if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+  for (int i = 0; i < 5; i++)
+    if (e->gev[i]) (void)hipEventDestroy(e->gev[i]);
+  if (e->st) (void)hipStreamDestroy(e->st);
+  if (e->st2) (void)hipStreamDestroy(e->st2);
   delete e;
 }
 
@@ -621,8 +734,9 @@ static double span_d(u64 lo, u64 hi) { return hi ? (double)(hi - lo) : 184467440
 // pi(x + y) - pi(x) < 2 y / ln y for y > 1), plus slack for tiny y.
 static u64 prime_bound(double y) { return y < 100 ? 64 : (u64)(2.0 * y / std::log(y)) + 64; }
 
-// Build the segment descriptors of [lo, hi) into segs.
-static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vector<SegDesc>& segs) {
+// Segment descriptors of [lo, hi) and the runs (<= RUN_MAX consecutive segments) they form.
+static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vector<SegDesc>& segs,
+                           std::vector<RunDesc>& runs) {
   const u32 SB = e->cfg.seg_bytes;
   const u64 span = 30ull * SB;
   u64 base = lo - lo % 30;
@@ -630,7 +744,6 @@ static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vec
     SegDesc d;
     d.base = base;
     d.rid = rid;
-    // numbers of this segment: [base, base + span) capped at hi (or 2^64)
     const u64 room = 0 - base;  // 2^64 - base (0 if base == 0: unlimited)
     u64 len = span;
     if (base != 0 && room < len) len = room;
@@ -640,12 +753,23 @@ static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vec
     d.lo_off = lo > base ? (u32)(lo - base) : 0;
     if (base == 0 && d.lo_off < 2) d.lo_off = 2;   // 1 is not prime
     d.hi_off = (u32)len;
-    // sieving primes needed: q^2 <= last number of the segment
     const u64 last = base + (len - 1);
     const u64 r = isqrt_u64(last);
     d.nq = (u32)(std::upper_bound(e->h_sp.begin(), e->h_sp.end(), (u32)std::min<u64>(r, 0xFFFFFFFFull)) -
                  e->h_sp.begin());
     d.pad = 0;
+    if (runs.empty() || runs.back().seg0 + runs.back().nseg != segs.size() || runs.back().nseg == RUN_MAX ||
+        segs.empty() || segs.back().rid != rid || segs.back().base + span != base) {
+      RunDesc rd;
+      rd.base = base;
+      rd.seg0 = (u32)segs.size();
+      rd.nseg = 0;
+      rd.nq = 0;
+      rd.pad = 0;
+      runs.push_back(rd);
+    }
+    runs.back().nseg++;
+    runs.back().nq = std::max(runs.back().nq, d.nq);
     segs.push_back(d);
     if (len < span) break;
     base += span;
@@ -654,52 +778,192 @@ static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vec
   }
 }
 
-// Sieve and test a list of pieces (all of one arithmetic class) whose total prime count
-// fits the prime buffer; accumulators indexed by Piece::rid.
-static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith, TestOut& to, float& sieve_ms,
-                      float& test_ms) {
+struct PassPlan {
   std::vector<SegDesc> segs;
-  for (const Piece& p : pieces) build_segments(e, p.lo, p.hi, p.rid, segs);
-  if (segs.empty()) return WF_OK;
-  int rc;
-  if ((rc = grow(e->d_segs, e->seg_cap, segs.size())) != WF_OK) return rc;
-  u64 bound = 0;
-  for (const Piece& p : pieces) bound += prime_bound(span_d(p.lo, p.hi));
-  if ((rc = grow(e->d_primes, e->prime_cap, bound)) != WF_OK) return rc;
-  if ((rc = grow(e->d_rid, e->rid_cap, e->prime_cap)) != WF_OK) return rc;
-  HIPCHK(hipMemcpyAsync(e->d_segs, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, e->st));
-  HIPCHK(hipMemsetAsync(e->d_ctr, 0, sizeof(unsigned long long), e->st));
-  const u32 SB = e->cfg.seg_bytes;
-  const int cus = e->prop.multiProcessorCount;
-  const u32 nseg = (u32)segs.size();
-  const u32 grid = std::min<u32>(nseg, (u32)cus * 4);
-  HIPCHK(hipEventRecord(e->ev[0], e->st));
-  hipLaunchKernelGGL(k_sieve, dim3(grid), dim3(e->cfg.sieve_block), SB, e->st, e->d_segs, nseg, e->d_sp,
-                     e->d_spinv, e->tierA, e->d_primes, e->d_rid, e->d_ctr, e->prime_cap, to.acc);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(e->ev[1], e->st));
-  const u32 tgrid = (u32)std::min<u64>((bound + 255) / 256, (u64)cus * 64);
-  switch (arith) {
-    case A_F64: hipLaunchKernelGGL(k_test_f64<5>, dim3(tgrid), dim3(256), 0, e->st, e->d_primes, e->d_rid, e->d_ctr, e->prime_cap, to); break;
-    case A_M2: hipLaunchKernelGGL((k_test_mont<2, false>), dim3(tgrid), dim3(256), 0, e->st, e->d_primes, e->d_rid, e->d_ctr, e->prime_cap, to); break;
-    case A_M3: hipLaunchKernelGGL((k_test_mont<3, false>), dim3(tgrid), dim3(256), 0, e->st, e->d_primes, e->d_rid, e->d_ctr, e->prime_cap, to); break;
-    case A_M4: hipLaunchKernelGGL((k_test_mont<4, false>), dim3(tgrid), dim3(256), 0, e->st, e->d_primes, e->d_rid, e->d_ctr, e->prime_cap, to); break;
-    default: hipLaunchKernelGGL((k_test_mont<4, true>), dim3(tgrid), dim3(256), 0, e->st, e->d_primes, e->d_rid, e->d_ctr, e->prime_cap, to); break;
+  std::vector<RunDesc> runs;
+  u32 maxnq = 0, bcap = 0, pcap = 0, tps = 0;
+};
+
+static void plan_pass(const wf_engine* e, const std::vector<Piece>& pieces, PassPlan& P) {
+  P.segs.clear();
+  P.runs.clear();
+  for (const Piece& p : pieces) build_segments(e, p.lo, p.hi, p.rid, P.segs, P.runs);
+  P.maxnq = 0;
+  for (const RunDesc& r : P.runs) P.maxnq = std::max(P.maxnq, r.nq);
+  P.pcap = (u32)prime_bound(30.0 * e->cfg.seg_bytes);
+  P.tps = (P.pcap + TEST_TILE - 1) / TEST_TILE;
+  P.bcap = 0;
+  if (P.maxnq > e->q1idx) {
+    // expected records per segment: sum over large primes and classes of seg_bytes / q; the
+    // count is a sum of independent 0/1 (or floor/ceil) terms, so its deviation is ~sqrt(E)
+    const double E = 8.0 * e->cfg.seg_bytes * (e->h_inv_sum[P.maxnq] - e->h_inv_sum[e->q1idx]);
+    P.bcap = (u32)((E + 10.0 * std::sqrt(E) + 1024.0 + 63.0) / 64.0) * 64;
   }
+}
+
+// Make a buffer set large enough for pass P.
+static int size_set(BufSet& B, const PassPlan& P) {
+  const u64 nseg = P.segs.size();
+  int rc;
+  if ((rc = grow(B.cnt, B.cnt_cap, nseg)) != WF_OK) return rc;
+  if ((rc = grow(B.off, B.off_cap, nseg * P.pcap)) != WF_OK) return rc;
+  if (P.bcap) {
+    if ((rc = grow(B.bcnt, B.bcnt_cap, nseg)) != WF_OK) return rc;
+    if ((rc = grow(B.bdata, B.bdata_cap, nseg * (u64)P.bcap)) != WF_OK) return rc;
+  }
+  return WF_OK;
+}
+
+// Bucket fill + sieve of pass P (descriptors at dsegs / druns) into buffer set B on stream st.
+static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, const RunDesc* druns, BufSet& B,
+                         hipStream_t st, DevAcc* acc) {
+  const u64 nseg = P.segs.size();
+  const int cus = e->prop.multiProcessorCount;
+  if (P.bcap) {
+    hipLaunchKernelGGL(k_zero, dim3((u32)std::min<u64>((nseg + 255) / 256, 1024)), dim3(256), 0, st, B.bcnt, nseg);
+    const u32 npb = (P.maxnq - e->q1idx + BF_PRIMES - 1) / BF_PRIMES;
+    hipLaunchKernelGGL(k_bucket_fill, dim3((u32)(P.runs.size() * npb)), dim3(256), 0, st, druns, npb, e->d_sp,
+                       e->d_spinv, e->q1idx, B.bcnt, B.bdata, P.bcap, e->sbshift, e->d_flags);
+    HIPCHK(hipGetLastError());
+  }
+  const u32 sgrid = (u32)std::min<u64>(nseg, (u64)cus * 4);
+  hipLaunchKernelGGL(k_sieve, dim3(sgrid), dim3(e->cfg.sieve_block), e->cfg.seg_bytes, st, dsegs, (u32)nseg, e->d_sp,
+                     e->d_spinv, e->tierA, e->q1idx, B.bcnt, B.bdata, P.bcap, B.off, B.cnt, P.pcap, acc, e->d_flags);
   HIPCHK(hipGetLastError());
+  return WF_OK;
+}
+
+// Fermat-quotient tests of pass P's primes (buffer set B) on stream st.
+static int enqueue_test(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, BufSet& B, hipStream_t st, int arith,
+                        const TestOut& to) {
+  const u64 nseg = P.segs.size();
+  const u32 tgrid = (u32)std::min<u64>(nseg * P.tps, (u64)e->prop.multiProcessorCount * 16);
+#define WF_TEST(A) hipLaunchKernelGGL(k_test<A>, dim3(tgrid), dim3(256), 0, st, dsegs, (u32)nseg, B.off, B.cnt, \
+                                      P.pcap, P.tps, to)
+  switch (arith) {
+    case A_F64: WF_TEST(A_F64); break;
+    case A_M2: WF_TEST(A_M2); break;
+    case A_M3: WF_TEST(A_M3); break;
+    case A_M4: WF_TEST(A_M4); break;
+    default: WF_TEST(A_FULL); break;
+  }
+#undef WF_TEST
+  HIPCHK(hipGetLastError());
+  return WF_OK;
+}
+
+// Upload the descriptors of pass P to the start of the descriptor arrays.
+static int upload_pass(wf_engine* e, const PassPlan& P) {
+  int rc;
+  if ((rc = grow(e->d_segs, e->seg_cap, P.segs.size())) != WF_OK) return rc;
+  if ((rc = grow(e->d_runs, e->run_cap, std::max<size_t>(P.runs.size(), 1))) != WF_OK) return rc;
+  HIPCHK(hipMemcpyAsync(e->d_segs, P.segs.data(), P.segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, e->st));
+  HIPCHK(hipMemcpyAsync(e->d_runs, P.runs.data(), P.runs.size() * sizeof(RunDesc), hipMemcpyHostToDevice, e->st));
+  return WF_OK;
+}
+
+// Plain path: sieve and test one pass on the engine's stream, then wait for it.
+static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith, const TestOut& to) {
+  PassPlan P;
+  plan_pass(e, pieces, P);
+  if (P.segs.empty()) return WF_OK;
+  int rc;
+  if ((rc = size_set(e->bs[0], P)) != WF_OK) return rc;
+  if ((rc = upload_pass(e, P)) != WF_OK) return rc;
+  HIPCHK(hipEventRecord(e->ev[0], e->st));
+  if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->bs[0], e->st, to.acc)) != WF_OK) return rc;
+  HIPCHK(hipEventRecord(e->ev[1], e->st));
+  if ((rc = enqueue_test(e, P, e->d_segs, e->bs[0], e->st, arith, to)) != WF_OK) return rc;
   HIPCHK(hipEventRecord(e->ev[2], e->st));
-  // check the prime-buffer bound (a proven bound; a violation is a bug)
-  unsigned long long appended = 0;
-  HIPCHK(hipMemcpyAsync(&appended, e->d_ctr, sizeof appended, hipMemcpyDeviceToHost, e->st));
+  u32 flags = 0;
+  HIPCHK(hipMemcpyAsync(&flags, e->d_flags, sizeof flags, hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipStreamSynchronize(e->st));
-  if (appended > e->prime_cap) return WF_E_OVERFLOW;
+  if (flags) return WF_E_OVERFLOW;
   float a = 0, b = 0;
-  hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
-  hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
-  sieve_ms += a;
-  test_ms += b;
-  e->last.tested += appended;
-  e->last.launches += 2;
+  (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
+  (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
+  e->last.sieve_ms += a;
+  e->last.test_ms += b;
+  e->last.launches += 1;
+  return WF_OK;
+}
+
+// Batched path: every pass of the batch in one HIP graph.  Pass k sieves into buffer set
+// k & 1 on the engine's stream while the test kernels of pass k - 1 run on the second
+// stream; a set is refilled only after the tests of the pass two back have finished.
+static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Piece>, int>>& passes,
+                       const TestOut& to) {
+  std::vector<PassPlan> plans(passes.size());
+  std::vector<u64> soff(passes.size()), roff(passes.size());
+  u64 nsegs = 0, nruns = 0;
+  for (size_t k = 0; k < passes.size(); k++) {
+    plan_pass(e, passes[k].first, plans[k]);
+    soff[k] = nsegs;
+    roff[k] = nruns;
+    nsegs += plans[k].segs.size();
+    nruns += plans[k].runs.size();
+  }
+  if (nsegs == 0) return WF_OK;
+  int rc;
+  if ((rc = grow(e->d_segs, e->seg_cap, nsegs)) != WF_OK) return rc;
+  if ((rc = grow(e->d_runs, e->run_cap, std::max<u64>(nruns, 1))) != WF_OK) return rc;
+  for (size_t k = 0; k < plans.size(); k++) {
+    if ((rc = size_set(e->bs[k & 1], plans[k])) != WF_OK) return rc;
+    HIPCHK(hipMemcpyAsync(e->d_segs + soff[k], plans[k].segs.data(), plans[k].segs.size() * sizeof(SegDesc),
+                          hipMemcpyHostToDevice, e->st));
+    if (!plans[k].runs.empty())
+      HIPCHK(hipMemcpyAsync(e->d_runs + roff[k], plans[k].runs.data(), plans[k].runs.size() * sizeof(RunDesc),
+                            hipMemcpyHostToDevice, e->st));
+  }
+  HIPCHK(hipStreamSynchronize(e->st));
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeRelaxed));
+  rc = WF_OK;
+  if (hipEventRecord(e->gev[0], e->st) != hipSuccess || hipStreamWaitEvent(e->st2, e->gev[0], 0) != hipSuccess)
+    rc = WF_E_DEVICE;
+  for (size_t k = 0; k < plans.size() && rc == WF_OK; k++) {
+    const PassPlan& P = plans[k];
+    if (P.segs.empty()) continue;
+    const int set = (int)(k & 1);
+    hipEvent_t sieved = e->gev[1 + set], tested = e->gev[3 + set];
+    if (k >= 2 && hipStreamWaitEvent(e->st, tested, 0) != hipSuccess) { rc = WF_E_DEVICE; break; }
+    if ((rc = enqueue_sieve(e, P, e->d_segs + soff[k], e->d_runs + roff[k], e->bs[set], e->st, to.acc)) != WF_OK) break;
+    if (hipEventRecord(sieved, e->st) != hipSuccess || hipStreamWaitEvent(e->st2, sieved, 0) != hipSuccess) {
+      rc = WF_E_DEVICE;
+      break;
+    }
+    if ((rc = enqueue_test(e, P, e->d_segs + soff[k], e->bs[set], e->st2, passes[k].second, to)) != WF_OK) break;
+    if (hipEventRecord(tested, e->st2) != hipSuccess) { rc = WF_E_DEVICE; break; }
+  }
+  // join the test stream back into the capturing stream
+  if (rc == WF_OK && (hipEventRecord(e->gev[0], e->st2) != hipSuccess ||
+                      hipStreamWaitEvent(e->st, e->gev[0], 0) != hipSuccess))
+    rc = WF_E_DEVICE;
+  const hipError_t ce = hipStreamEndCapture(e->st, &graph);
+  if (rc != WF_OK || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc != WF_OK ? rc : WF_E_DEVICE;
+  }
+  hipGraphExec_t exec = nullptr;
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return WF_E_DEVICE;
+  }
+  u32 flags = 0;
+  rc = WF_OK;
+  if (hipEventRecord(e->ev[0], e->st) != hipSuccess || hipGraphLaunch(exec, e->st) != hipSuccess ||
+      hipEventRecord(e->ev[1], e->st) != hipSuccess ||
+      hipMemcpyAsync(&flags, e->d_flags, sizeof flags, hipMemcpyDeviceToHost, e->st) != hipSuccess ||
+      hipStreamSynchronize(e->st) != hipSuccess)
+    rc = WF_E_DEVICE;
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (rc != WF_OK) return rc;
+  if (flags) return WF_E_OVERFLOW;
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
+  e->last.total_ms += ms;
+  e->last.launches += 1;
   return WF_OK;
 }
 
@@ -710,7 +974,7 @@ static bool cmp_dev_event(const DevEvent& a, const DevEvent& b) {
 
 // Scan n ranges; per-range totals into res[0..n), events into ev.
 static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_result* res, wf_event* ev,
-                       u64 ev_cap) {
+                       u64 ev_cap, bool batched) {
   if (!e || !r || !res || n == 0) return WF_E_ARG;
   if (e->nb != 1 || e->bases[0] != 2) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
@@ -726,7 +990,8 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   const u64 want_ev = std::max<u64>(ev_cap, 1 << 16);
   if ((rc = grow(e->d_ev, e->dev_ev_cap, want_ev)) != WF_OK) return rc;
   HIPCHK(hipMemsetAsync(e->d_acc, 0, n * sizeof(DevAcc), e->st));
-  HIPCHK(hipMemsetAsync(e->d_ctr + 1, 0, sizeof(unsigned long long), e->st));
+  HIPCHK(hipMemsetAsync(e->d_ctr, 0, 2 * sizeof(unsigned long long), e->st));
+  HIPCHK(hipMemsetAsync(e->d_flags, 0, sizeof(u32), e->st));
   TestOut to;
   to.acc = e->d_acc;
   to.ev = e->d_ev;
@@ -737,10 +1002,10 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   // pieces by arithmetic class, then passes of at most cfg.chunk numbers each
   std::vector<Piece> all;
   for (u32 i = 0; i < n; i++) split_classes(e, r[i].lo, r[i].hi, i, all);
-  float sieve_ms = 0, test_ms = 0;
   const double chunk = (double)e->cfg.chunk;
   const u64 segspan = 30ull * e->cfg.seg_bytes;
   const u64 step = std::max<u64>(1, (u64)(chunk / (double)segspan)) * segspan;
+  std::vector<std::pair<std::vector<Piece>, int>> passes;
   for (int cls = A_F64; cls <= A_FULL; cls++) {
     std::vector<Piece> batch;
     double in_batch = 0;
@@ -755,7 +1020,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
         part.lo = lo;
         part.hi = last ? p0.hi : lo + step;
         if (in_batch + take > chunk && !batch.empty()) {
-          if ((rc = run_pieces(e, batch, cls, to, sieve_ms, test_ms)) != WF_OK) return rc;
+          passes.push_back({batch, cls});
           batch.clear();
           in_batch = 0;
         }
@@ -765,7 +1030,13 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
         lo += step;
       }
     }
-    if (!batch.empty() && (rc = run_pieces(e, batch, cls, to, sieve_ms, test_ms)) != WF_OK) return rc;
+    if (!batch.empty()) passes.push_back({batch, cls});
+  }
+  if (batched) {
+    if ((rc = run_batched(e, passes, to)) != WF_OK) return rc;
+  } else {
+    for (auto& ps : passes)
+      if ((rc = run_pieces(e, ps.first, ps.second, to)) != WF_OK) return rc;
   }
   // results
   std::vector<DevAcc> acc(n);
@@ -777,6 +1048,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   std::vector<DevEvent> dev(got);
   if (got) HIPCHK(hipMemcpy(dev.data(), e->d_ev, got * sizeof(DevEvent), hipMemcpyDeviceToHost));
   std::sort(dev.begin(), dev.end(), cmp_dev_event);
+  u64 tested = 0;
   for (u32 i = 0; i < n; i++) {
     wf_result& o = res[i];
     memset(&o, 0, sizeof o);
@@ -789,6 +1061,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
     o.cks[0] = acc[i].cks;
     o.sumq[0] = acc[i].sumq;
     o.nev = acc[i].nev;
+    tested += o.tests;
   }
   for (u64 k = 0; k < got && k < ev_cap; k++) {
     ev[k].type = dev[k].type;
@@ -798,9 +1071,8 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
     ev[k].r_hi = dev[k].r_hi;
     ev[k].r_lo = dev[k].r_lo;
   }
-  e->last.sieve_ms = sieve_ms;
-  e->last.test_ms = test_ms;
-  e->last.total_ms = sieve_ms + test_ms;
+  e->last.tested = tested;
+  if (!batched) e->last.total_ms = e->last.sieve_ms + e->last.test_ms;
   if (nev > e->dev_ev_cap) return WF_E_OVERFLOW;
   return WF_OK;
 }
@@ -812,13 +1084,13 @@ extern "C" int wf_scan(wf_engine* e, u64 lo, u64 hi, int64_t near_T, wf_result* 
   (void)psp;
   (void)psp_cap;
   wf_range r = {lo, hi};
-  return scan_ranges(e, &r, 1, near_T, out, ev, ev_cap);
+  return scan_ranges(e, &r, 1, near_T, out, ev, ev_cap, false);
 }
 
 extern "C" int wf_scan_batch(wf_engine* e, const wf_range* r, u32 n, int64_t near_T, wf_result* per_range,
                              wf_event* ev, u64 ev_cap) {
   if (!e || !r || !per_range || (ev_cap && !ev)) return WF_E_ARG;
-  return scan_ranges(e, r, n, near_T, per_range, ev, ev_cap);
+  return scan_ranges(e, r, n, near_T, per_range, ev, ev_cap, true);
 }
 
 extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
@@ -827,7 +1099,7 @@ extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* 
   for (u64 i = 0; i < n; i++)
     if (p[i] < 3 || !(p[i] & 1)) return WF_E_ARG;
   HIPCHK(hipSetDevice(e->dev));
-  u64 *d = nullptr;
+  u64* d = nullptr;
   if (hipMalloc(&d, 4 * n * sizeof(u64)) != hipSuccess) return WF_E_NOMEM;
   int rc = WF_OK;
   if (hipMemcpyAsync(d, p, n * sizeof(u64), hipMemcpyHostToDevice, e->st) != hipSuccess) rc = WF_E_DEVICE;
@@ -840,7 +1112,7 @@ extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* 
                       hipMemcpyAsync(r_lo, d + 3 * n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st) != hipSuccess ||
                       hipStreamSynchronize(e->st) != hipSuccess))
     rc = WF_E_DEVICE;
-  hipFree(d);
+  (void)hipFree(d);
   return rc;
 }
 
@@ -852,29 +1124,103 @@ extern "C" int wf_primes(wf_engine* e, u64 lo, u64 hi, u64* out, u64 cap, u64* c
   if ((rc = ensure_sieving_primes(e, std::max<u64>(isqrt_u64(top) + 1, 64))) != WF_OK) return rc;
   if ((rc = grow(e->d_acc, e->acc_cap, 1)) != WF_OK) return rc;
   HIPCHK(hipMemsetAsync(e->d_acc, 0, sizeof(DevAcc), e->st));
-  std::vector<SegDesc> segs;
-  build_segments(e, lo, hi, 0, segs);
-  if ((rc = grow(e->d_segs, e->seg_cap, segs.size())) != WF_OK) return rc;
-  const u64 bound = prime_bound(span_d(lo, hi));
-  if ((rc = grow(e->d_primes, e->prime_cap, bound)) != WF_OK) return rc;
-  if ((rc = grow(e->d_rid, e->rid_cap, e->prime_cap)) != WF_OK) return rc;
-  HIPCHK(hipMemcpyAsync(e->d_segs, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, e->st));
-  HIPCHK(hipMemsetAsync(e->d_ctr, 0, sizeof(unsigned long long), e->st));
-  const u32 nseg = (u32)segs.size();
-  hipLaunchKernelGGL(k_sieve, dim3(std::min<u32>(nseg, (u32)e->prop.multiProcessorCount * 4)),
-                     dim3(e->cfg.sieve_block), e->cfg.seg_bytes, e->st, e->d_segs, nseg, e->d_sp, e->d_spinv,
-                     e->tierA, e->d_primes, e->d_rid, e->d_ctr, e->prime_cap, e->d_acc);
-  HIPCHK(hipGetLastError());
-  unsigned long long n = 0;
-  HIPCHK(hipMemcpyAsync(&n, e->d_ctr, sizeof n, hipMemcpyDeviceToHost, e->st));
-  HIPCHK(hipStreamSynchronize(e->st));
-  if (n > e->prime_cap) return WF_E_OVERFLOW;
-  std::vector<u64> all(n);
-  if (n) HIPCHK(hipMemcpy(all.data(), e->d_primes, n * sizeof(u64), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemsetAsync(e->d_flags, 0, sizeof(u32), e->st));
+  std::vector<u64> all;
   if (lo <= 2 && (hi == 0 || hi > 2)) all.push_back(2);
+  // passes of at most cfg.chunk numbers, sieve only
+  const u64 segspan = 30ull * e->cfg.seg_bytes;
+  const u64 step = std::max<u64>(1, e->cfg.chunk / segspan) * segspan;
+  u64 cur = lo;
+  for (;;) {
+    const bool last = span_d(cur, hi) <= (double)step;
+    std::vector<Piece> pieces = {{cur, last ? hi : cur + step, 0, A_F64}};
+    PassPlan P;
+    plan_pass(e, pieces, P);
+    if ((rc = size_set(e->bs[0], P)) != WF_OK) return rc;
+    if ((rc = upload_pass(e, P)) != WF_OK) return rc;
+    if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->bs[0], e->st, e->d_acc)) != WF_OK) return rc;
+    std::vector<u32> cnt(P.segs.size());
+    u32 flags = 0;
+    HIPCHK(hipMemcpyAsync(cnt.data(), e->bs[0].cnt, cnt.size() * sizeof(u32), hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipMemcpyAsync(&flags, e->d_flags, sizeof flags, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (flags) return WF_E_OVERFLOW;
+    std::vector<u32> offs;
+    for (size_t s = 0; s < P.segs.size(); s++) {
+      offs.resize(cnt[s]);
+      if (cnt[s]) HIPCHK(hipMemcpy(offs.data(), e->bs[0].off + s * P.pcap, cnt[s] * sizeof(u32), hipMemcpyDeviceToHost));
+      for (u32 k = 0; k < cnt[s]; k++) all.push_back(P.segs[s].base + offs[k]);
+    }
+    if (last) break;
+    cur += step;
+  }
   std::sort(all.begin(), all.end());
   *count = all.size();
   for (u64 i = 0; i < all.size() && i < cap; i++) out[i] = all[i];
+  return WF_OK;
+}
+
+// Switch the engine to configuration c (segment size, sieve workgroup, bucket threshold).
+static void apply_config(wf_engine* e, const wf_config& c) {
+  e->cfg.seg_bytes = c.seg_bytes;
+  e->cfg.sieve_block = c.sieve_block;
+  e->cfg.bucket_min = c.bucket_min;
+  e->sbshift = __builtin_ctz(e->cfg.seg_bytes);
+  if (!e->h_sp.empty())
+    e->q1idx = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), e->cfg.bucket_min) - e->h_sp.begin());
+}
+
+// Autotuner: time the sieve stage (bucket fill + segment sieve + extraction) of a sample
+// of [lo, hi) for every segment size 16-64 KiB, sieve workgroup of 4-16 wavefronts and
+// bucket threshold 2^15-2^18, keep the fastest.  The test kernel does not depend on them.
+extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double* best_ms) {
+  if (!e || (hi != 0 && hi <= lo)) return WF_E_ARG;
+  HIPCHK(hipSetDevice(e->dev));
+  const u64 top = hi ? hi - 1 : 0xFFFFFFFFFFFFFFFFull;
+  int rc;
+  if ((rc = ensure_sieving_primes(e, std::max<u64>(isqrt_u64(top) + 1, 64))) != WF_OK) return rc;
+  if ((rc = grow(e->d_acc, e->acc_cap, 1)) != WF_OK) return rc;
+  // sample: up to 3e9 numbers from the start of the range
+  const double whole = span_d(lo, hi);
+  const u64 sample = (u64)std::min(whole, 3.0e9);
+  const wf_config orig = e->cfg;
+  wf_config bestc = orig;
+  double bestt = 1e300;
+  const u32 segs[] = {16384, 32768, 65536}, blocks[] = {256, 512, 1024}, bmins[] = {32768, 65536, 131072, 262144};
+  for (u32 sb : segs)
+    for (u32 bl : blocks)
+      for (u32 bm : bmins) {
+        if (sb + 64 > e->prop.sharedMemPerBlock) continue;
+        wf_config c = orig;
+        c.seg_bytes = sb;
+        c.sieve_block = bl;
+        c.bucket_min = bm;
+        apply_config(e, c);
+        std::vector<Piece> pieces = {{lo, lo + sample, 0, A_F64}};
+        PassPlan P;
+        plan_pass(e, pieces, P);
+        if ((rc = size_set(e->bs[0], P)) != WF_OK) { apply_config(e, orig); return rc; }
+        if ((rc = upload_pass(e, P)) != WF_OK) { apply_config(e, orig); return rc; }
+        HIPCHK(hipMemsetAsync(e->d_flags, 0, sizeof(u32), e->st));
+        float ms = 0;
+        for (int rep = 0; rep < 2; rep++) {   // the first run warms caches and code
+          HIPCHK(hipEventRecord(e->ev[0], e->st));
+          if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->bs[0], e->st, e->d_acc)) != WF_OK) {
+            apply_config(e, orig);
+            return rc;
+          }
+          HIPCHK(hipEventRecord(e->ev[1], e->st));
+          HIPCHK(hipStreamSynchronize(e->st));
+          (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
+        }
+        if (ms < bestt) {
+          bestt = ms;
+          bestc = c;
+        }
+      }
+  apply_config(e, bestc);
+  if (best) *best = e->cfg;
+  if (best_ms) *best_ms = bestt / ((double)sample / 1e9);   // ms per 1e9 numbers
   return WF_OK;
 }
 

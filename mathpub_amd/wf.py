@@ -29,7 +29,8 @@ class Range(ctypes.Structure):
 
 
 class Config(ctypes.Structure):
-    _fields_ = [("seg_bytes", u32), ("sieve_block", u32), ("test_block", u32), ("arith", u32), ("chunk", u64)]
+    _fields_ = [("seg_bytes", u32), ("sieve_block", u32), ("test_block", u32), ("arith", u32), ("bucket_min", u32),
+                ("chunk", u64)]
 
 
 class Timing(ctypes.Structure):
@@ -69,12 +70,13 @@ def load():
     L.wf_quotients.argtypes = [ctypes.c_void_p, P64, P64, P64, P64, u64]
     L.wf_primes.argtypes = [ctypes.c_void_p, u64, u64, P64, u64, P64]
     L.wf_last_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(Timing)]
+    L.wf_autotune.argtypes = [ctypes.c_void_p, u64, u64, ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_double)]
     L.wf_serve_start.argtypes = [ctypes.c_void_p, u32, u64]
     L.wf_serve_submit.argtypes = [ctypes.c_void_p, u64, u64, P64]
     L.wf_serve_poll.argtypes = [ctypes.c_void_p, u64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_int)]
     L.wf_serve_stop.argtypes = [ctypes.c_void_p]
     for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_primes",
-              "wf_last_timing", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
+              "wf_last_timing", "wf_autotune", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
         getattr(L, f).restype = ctypes.c_int
     _lib = L
     return L
@@ -102,9 +104,9 @@ def result_dict(res, events=None):
 class Engine:
     """One GPU engine (wf_open); not thread-safe, like the C object."""
 
-    def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0):
+    def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0, bucket_min=0):
         L = load()
-        cfg = Config(seg_bytes, sieve_block, 0, arith, chunk)
+        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk)
         h = ctypes.c_void_p()
         _check(L.wf_open(device, ctypes.byref(cfg), ctypes.byref(h)), "wf_open")
         self._h = h
@@ -170,6 +172,13 @@ class Engine:
         buf = (u64 * max(cnt.value, 1))()
         _check(self._L.wf_primes(self._h, lo, _hi(hi), buf, cnt.value, ctypes.byref(cnt)), "wf_primes")
         return list(buf[:cnt.value])
+
+    def autotune(self, lo, hi):
+        """Pick segment size, sieve workgroup and bucket threshold for [lo, hi) (wf_autotune)."""
+        c, ms = Config(), ctypes.c_double()
+        _check(self._L.wf_autotune(self._h, lo, _hi(hi), ctypes.byref(c), ctypes.byref(ms)), "wf_autotune")
+        return {"seg_bytes": c.seg_bytes, "sieve_block": c.sieve_block, "bucket_min": c.bucket_min,
+                "sieve_ms_per_1e9": ms.value}
 
     def last_timing(self):
         t = Timing()
