@@ -88,6 +88,131 @@ def c3_ranges(n=4096, seed=0x243F6A8885A308D3):
     return out
 
 
+def _engine_line(metric, value, unit, ms, args, config, extra=None):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+           "data": "synthetic (ranges from a fixed splitmix64 seed; primes sieved on the GPU)", "config": config,
+           "roofline": None, "cpu_baseline": None}
+    if extra:
+        out.update(extra)
+    print(json.dumps(out))
+    return 0
+
+
+def bench_c3(args):
+    """4096 uneven ranges in one batched (HIP graph) call, per-range counts and checksums."""
+    from mathpub_amd import wf
+    ranges = c3_ranges()
+    eng = wf.Engine(arith=args.arith)
+    t = time.perf_counter()
+    tuned = None if args.no_tune else eng.autotune(ranges[0][0], ranges[0][1])
+    print("autotune %s (%.1f s)" % (tuned, time.perf_counter() - t), file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        per, ev = eng.scan_batch(ranges)
+        print("c3 warmup %d: %.2f s" % (i, time.perf_counter() - t), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        per, ev = eng.scan_batch(ranges)
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    tests = sum(r["tests"] for r in per)
+    numbers = sum(h - l for l, h in ranges)
+    prods = sum(r["tests"] * wf_products(((l + h) // 2).bit_length()) for (l, h), r in zip(ranges, per))
+    digest = 0
+    for r in per:
+        digest = (digest * 0x100000001B3 ^ r["cks"] ^ r["primes"]) & ((1 << 64) - 1)
+    eng.close()
+    return _engine_line("primes Wieferich-tested/sec, 4096 batched ranges", tests / (ms / 1e3), "primes tested/s",
+                        ms, args, {"workload": "4096 ranges, starts U[1e12,1e18], lengths logU[1e6,1e9], seed "
+                                   "0x243F6A8885A308D3", "engine": "batched", "ranges": len(ranges),
+                                   "numbers": numbers, "tests": tests, "per_range_digest": "%016x" % digest,
+                                   "autotuned": tuned, "w_products_per_s": prods / (ms / 1e3)})
+
+
+def bench_c4(args):
+    """A slice [2^64 - S, 2^64) of the top-of-word range: full 128-bit Montgomery with carry
+    plus the base-2 strong-pseudoprime flags of every odd composite."""
+    from mathpub_amd import wf
+    span = int(args.c4_span)
+    lo = (1 << 64) - span
+    eng = wf.Engine()
+    for i in range(args.warmup):
+        eng.scan(lo, 1 << 64, psp=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = eng.scan(lo, 1 << 64, psp=True)
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    tm = eng.last_timing()
+    eng.close()
+    return _engine_line("primes Wieferich-tested/sec at p~2^64 with SPRP-2 flags", r["tests"] / (ms / 1e3),
+                        "primes tested/s", ms, args,
+                        {"workload": "[2^64-%d, 2^64), base 2, strong pseudoprimes to base 2 listed" % span,
+                         "engine": "plain", "tests": r["tests"], "cks": "%016x" % r["cks"], "psp": r["psp"],
+                         "odd_numbers_per_s": span / 2 / (ms / 1e3), "sieve_ms": tm["sieve_ms"],
+                         "test_ms": tm["test_ms"]})
+
+
+def c5_queries(n, seed=0x243F6A8885A308D3 ^ 5):
+    """n range queries: lengths log-uniform in [1e5, 1e10], starts uniform in [1e9, 1e18]."""
+    import random
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        lo = rng.randrange(10 ** 9, 10 ** 18)
+        out.append((lo, lo + int(10 ** rng.uniform(5, 10))))
+    return out
+
+
+def bench_c5(args):
+    """Serving: Poisson arrivals at each rate, continuous batching in the library, latency
+    from arrival to completion (admission waits included)."""
+    import random
+    from mathpub_amd import wf
+    eng = wf.Engine()
+    eng.scan(10 ** 18, 10 ** 18 + 10 ** 6)          # device-resident sieving primes up to 1e9
+    qs = c5_queries(args.queries)
+    rng = random.Random(17)
+    sweep = []
+    for rate in [float(x) for x in args.rates.split(",")]:
+        eng.serve_start(slots=256, hbm_budget=48 << 30)
+        arrivals, t = [], 0.0
+        for _ in qs:
+            t += rng.expovariate(rate)
+            arrivals.append(t)
+        lat, live, nxt, done_tests, numbers = [], {}, 0, 0, 0
+        t0 = time.perf_counter()
+        while nxt < len(qs) or live:
+            now = time.perf_counter() - t0
+            while nxt < len(qs) and arrivals[nxt] <= now:
+                tk = eng.serve_submit(*qs[nxt])
+                if tk is None:
+                    break                                  # full: the request waits (its latency grows)
+                live[tk] = nxt
+                nxt += 1
+            for tk in list(live):
+                r = eng.serve_poll(tk)
+                if r is not None:
+                    i = live.pop(tk)
+                    lat.append(time.perf_counter() - t0 - arrivals[i])
+                    done_tests += r["tests"]
+                    numbers += qs[i][1] - qs[i][0]
+            time.sleep(0.0005)
+        wall = time.perf_counter() - t0
+        eng.serve_stop()
+        lat.sort()
+        sweep.append({"rate_per_s": rate, "queries": len(lat), "p50_s": lat[len(lat) // 2],
+                      "p99_s": lat[min(len(lat) - 1, int(len(lat) * 0.99))], "wall_s": wall,
+                      "primes_tested_per_s": done_tests / wall, "numbers_per_s": numbers / wall})
+        print("rate %g: %s" % (rate, sweep[-1]), file=sys.stderr, flush=True)
+    eng.close()
+    best = max(x["primes_tested_per_s"] for x in sweep)
+    return _engine_line("serving: primes Wieferich-tested/sec and p50/p99 latency", best, "primes tested/s",
+                        sweep[-1]["wall_s"] * 1e3, args,
+                        {"workload": "%d queries per rate, lengths logU[1e5,1e10], starts U[1e9,1e18], Poisson "
+                                     "arrivals" % len(qs), "engine": "serving (continuous batching)",
+                         "sweep": sweep})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +223,15 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
     ap.add_argument("--span", type=float, default=SPAN, help="numbers per step (default 1e11)")
     ap.add_argument("--mode", choices=["both", "plain", "batched"], default="both")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+                    help="c2 = the single range (default, the headline line); c3 = 4096 batched ranges; "
+                         "c4 = a slice of the top-of-word range with strong-pseudoprime flags; c5 = serving")
+    ap.add_argument("--queries", type=int, default=2000, help="c5: queries per rate")
+    ap.add_argument("--rates", default="100,1000,5000", help="c5: Poisson arrival rates per second")
+    ap.add_argument("--c4-span", type=float, default=2.0 ** 34, help="c4: numbers below 2^64 per step")
     args = ap.parse_args()
+    if args.workload != "c2" and args.gpus == 1:
+        return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
               "(ranges shard by splitting [lo, hi), see DESIGN.md)", file=sys.stderr)

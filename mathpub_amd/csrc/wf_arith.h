@@ -298,3 +298,57 @@ __device__ __forceinline__ void mont_pow2(u64 p, u64 e, int ebits, u32 (&r)[L]) 
 }
 
 }  // namespace wf
+
+namespace wf {
+
+// ---------------------------------------------------------------------------------
+// base-2 strong probable-prime test of an odd n >= 3, n < 2^64
+// ---------------------------------------------------------------------------------
+// Montgomery arithmetic mod n with R = 2^64 on 64-bit words (each product is built from
+// 32-bit multiplies).  n may exceed 2^63, so REDC keeps its carry bit and every result is
+// reduced to [0, n).
+
+// (a b + m n) / 2^64 mod n for a, b < n, ninv = -n^-1 mod 2^64
+__device__ __forceinline__ u64 mont64_mul(u64 a, u64 b, u64 n, u64 ninv) {
+  const u64 lo = a * b, hi = __umul64hi(a, b);
+  const u64 m = lo * ninv;
+  const u64 mh = __umul64hi(m, n);
+  // lo + m n has low word 0 and carries 1 unless lo == 0
+  const u64 t = hi + mh;
+  const bool c1 = t < hi;
+  const u64 r = t + (lo != 0);
+  const bool c2 = r < t;
+  return (c1 || c2 || r >= n) ? r - n : r;
+}
+
+__device__ __forceinline__ u64 mont64_dbl(u64 x, u64 n) {
+  const bool c = x >> 63;
+  const u64 y = x << 1;
+  return (c || y >= n) ? y - n : y;
+}
+
+// true iff n is a base-2 strong probable prime (n odd, n >= 3)
+__device__ __forceinline__ bool sprp2_64(u64 n) {
+  const u64 ninv = 0 - inv64(n);
+  const int s = __builtin_ctzll(n - 1);
+  const u64 d = (n - 1) >> s;
+  // one = R mod n: 2^(bits(n) - 1) doubled up to 2^64
+  const int nb = 64 - __builtin_clzll(n);
+  u64 one = 1ull << (nb - 1);
+  for (int k = nb - 1; k < 64; k++) one = mont64_dbl(one, n);
+  const u64 mone = n - one;                       // Montgomery form of n - 1
+  // x = 2^d in Montgomery form, left to right from the top bit of d
+  u64 x = mont64_dbl(one, n);                     // 2
+  for (int i = 62 - __builtin_clzll(d); i >= 0; i--) {
+    x = mont64_mul(x, x, n, ninv);
+    if ((d >> i) & 1) x = mont64_dbl(x, n);
+  }
+  if (x == one || x == mone) return true;
+  for (int r = 1; r < s; r++) {
+    x = mont64_mul(x, x, n, ninv);
+    if (x == mone) return true;
+  }
+  return false;
+}
+
+}  // namespace wf

@@ -20,6 +20,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "wf.h"
@@ -202,7 +206,8 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
                                                 const u32* __restrict__ bcnt, const u32* __restrict__ bdata,
                                                 u32 bcap,
                                                 u32* __restrict__ out, u32* __restrict__ cnt, u32 pcap,
-                                                DevAcc* __restrict__ acc, u32* __restrict__ flags) {
+                                                DevAcc* __restrict__ acc, u32* __restrict__ flags,
+                                                u32* __restrict__ bitmap, u32 bmw) {
   extern __shared__ u32 seg[];
   __shared__ u32 wtot[16];
   const int lane = threadIdx.x & 63;
@@ -249,6 +254,9 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
       }
     }
     __syncthreads();
+    // composite bitmap for the strong-pseudoprime pass
+    if (bitmap)
+      for (u32 w = threadIdx.x; w < nwords; w += blockDim.x) bitmap[(u64)s * bmw + w] = seg[w];
     // extraction: wave w owns words [w M, (w + 1) M); lanes read consecutive words
     const u32 M = (nwords + nwaves - 1) / nwaves;
     const u32 wb = min((u32)wave * M, nwords), we = min(wb + M, nwords);
@@ -445,6 +453,44 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
   rlo_out[i] = rlo;
 }
 
+// Base-2 strong pseudoprimes: every odd composite n of the segments (wheel composites
+// from the sieve bitmap, and the odd multiples of 3 and 5) gets the exact SPRP-2 test;
+// the ones that pass are appended to out[] and counted per range.
+#define PSP_TILE 1024
+__constant__ signed char c_W30[30] = {-1, 0, -1, -1, -1, -1, -1, 1, -1, -1, -1, 2, -1, 3, -1,
+                                      -1, -1, 4, -1, 5, -1, -1, -1, 6, -1, -1, -1, -1, -1, 7};
+
+__global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u32 nseg,
+                                             const u32* __restrict__ bitmap, u32 bmw, u32 tps,
+                                             u64* __restrict__ out, u64 cap, unsigned long long* __restrict__ count,
+                                             DevAcc* __restrict__ acc) {
+  const u64 ntiles = (u64)nseg * tps;
+  for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const u32 sg = (u32)(t / tps);
+    const u32 k = (u32)(t - (u64)sg * tps);
+    const SegDesc d = segs[sg];
+    const u32 nodd = 15u * d.nbytes;                 // odd offsets 1, 3, ..., 30 nbytes - 1
+    if (k * PSP_TILE >= nodd) continue;
+    for (u32 j = threadIdx.x; j < PSP_TILE; j += blockDim.x) {
+      const u32 i = k * PSP_TILE + j;
+      if (i >= nodd) break;
+      const u32 off = 2u * i + 1u;
+      if (off < d.lo_off || off >= d.hi_off) continue;
+      const u64 n = d.base + off;
+      bool comp;
+      const u32 r = off % 30u;
+      const int cls = c_W30[r];
+      if (cls < 0) comp = n > 5;                      // an odd multiple of 3 or 5
+      else comp = (bitmap[(u64)sg * bmw + off / 120u] >> (((off / 30u) & 3u) * 8u + (u32)cls)) & 1u;
+      if (comp && sprp2_64(n)) {
+        const unsigned long long kk = atomicAdd(count, 1ull);
+        if (kk < cap) out[kk] = n;
+        atomicAdd(&acc[d.rid].npsp, 1ull);
+      }
+    }
+  }
+}
+
 // Zero n words (a kernel, so that the batched path can record it into a HIP graph).
 __global__ void k_zero(u32* __restrict__ p, u64 n) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = 0;
@@ -465,6 +511,8 @@ struct BufSet {   // bucket counters / records, prime offsets / counts of one pa
   u64 off_cap = 0;
   u32* cnt = nullptr;
   u64 cnt_cap = 0;
+  u32* bitmap = nullptr;   // composite bitmap (strong-pseudoprime pass only)
+  u64 bitmap_cap = 0;
 };
 
 struct wf_engine {
@@ -495,9 +543,12 @@ struct wf_engine {
   u64 acc_cap = 0;
   DevEvent* d_ev = nullptr;
   u64 dev_ev_cap = 0;
-  unsigned long long* d_ctr = nullptr;   // [1] events
+  unsigned long long* d_ctr = nullptr;   // [1] events, [2] strong pseudoprimes
+  u64* d_psp = nullptr;                  // strong pseudoprimes found
+  u64 psp_cap = 0;
   hipEvent_t ev[4] = {};
   wf_timing last;
+  struct Server* srv = nullptr;
 };
 
 #define HIPCHK(x)                                  \
@@ -660,6 +711,7 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
 
 extern "C" void wf_close(wf_engine* e) {
   if (!e) return;
+  if (e->srv) (void)wf_serve_stop(e);
   (void)hipSetDevice(e->dev);
   if (e->st) (void)hipStreamSynchronize(e->st);
   (void)hipFree(e->d_sp);
@@ -671,7 +723,9 @@ extern "C" void wf_close(wf_engine* e) {
     (void)hipFree(e->bs[k].bdata);
     (void)hipFree(e->bs[k].off);
     (void)hipFree(e->bs[k].cnt);
+    (void)hipFree(e->bs[k].bitmap);
   }
+  (void)hipFree(e->d_psp);
   (void)hipFree(e->d_flags);
   (void)hipFree(e->d_acc);
   (void)hipFree(e->d_ev);
@@ -816,7 +870,7 @@ static int size_set(BufSet& B, const PassPlan& P) {
 
 // Bucket fill + sieve of pass P (descriptors at dsegs / druns) into buffer set B on stream st.
 static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, const RunDesc* druns, BufSet& B,
-                         hipStream_t st, DevAcc* acc) {
+                         hipStream_t st, DevAcc* acc, bool want_bitmap = false) {
   const u64 nseg = P.segs.size();
   const int cus = e->prop.multiProcessorCount;
   if (P.bcap) {
@@ -828,7 +882,8 @@ static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, 
   }
   const u32 sgrid = (u32)std::min<u64>(nseg, (u64)cus * 4);
   hipLaunchKernelGGL(k_sieve, dim3(sgrid), dim3(e->cfg.sieve_block), e->cfg.seg_bytes, st, dsegs, (u32)nseg, e->d_sp,
-                     e->d_spinv, e->tierA, e->q1idx, B.bcnt, B.bdata, P.bcap, B.off, B.cnt, P.pcap, acc, e->d_flags);
+                     e->d_spinv, e->tierA, e->q1idx, B.bcnt, B.bdata, P.bcap, B.off, B.cnt, P.pcap, acc, e->d_flags,
+                     want_bitmap ? B.bitmap : nullptr, e->cfg.seg_bytes / 4);
   HIPCHK(hipGetLastError());
   return WF_OK;
 }
@@ -863,18 +918,29 @@ static int upload_pass(wf_engine* e, const PassPlan& P) {
 }
 
 // Plain path: sieve and test one pass on the engine's stream, then wait for it.
-static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith, const TestOut& to) {
+static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith, const TestOut& to,
+                      bool want_psp = false) {
   PassPlan P;
   plan_pass(e, pieces, P);
   if (P.segs.empty()) return WF_OK;
   int rc;
   if ((rc = size_set(e->bs[0], P)) != WF_OK) return rc;
+  if (want_psp && (rc = grow(e->bs[0].bitmap, e->bs[0].bitmap_cap, P.segs.size() * (e->cfg.seg_bytes / 4))) != WF_OK)
+    return rc;
   if ((rc = upload_pass(e, P)) != WF_OK) return rc;
   HIPCHK(hipEventRecord(e->ev[0], e->st));
-  if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->bs[0], e->st, to.acc)) != WF_OK) return rc;
+  if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->bs[0], e->st, to.acc, want_psp)) != WF_OK) return rc;
   HIPCHK(hipEventRecord(e->ev[1], e->st));
   if ((rc = enqueue_test(e, P, e->d_segs, e->bs[0], e->st, arith, to)) != WF_OK) return rc;
   HIPCHK(hipEventRecord(e->ev[2], e->st));
+  if (want_psp) {
+    const u32 tps = (15u * e->cfg.seg_bytes + PSP_TILE - 1) / PSP_TILE;
+    const u64 nt = P.segs.size() * (u64)tps;
+    hipLaunchKernelGGL(k_psp, dim3((u32)std::min<u64>(nt, (u64)e->prop.multiProcessorCount * 16)), dim3(256), 0, e->st,
+                       e->d_segs, (u32)P.segs.size(), e->bs[0].bitmap, e->cfg.seg_bytes / 4, tps, e->d_psp,
+                       e->psp_cap, e->d_ctr + 2, to.acc);
+    HIPCHK(hipGetLastError());
+  }
   u32 flags = 0;
   HIPCHK(hipMemcpyAsync(&flags, e->d_flags, sizeof flags, hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipStreamSynchronize(e->st));
@@ -974,7 +1040,7 @@ static bool cmp_dev_event(const DevEvent& a, const DevEvent& b) {
 
 // Scan n ranges; per-range totals into res[0..n), events into ev.
 static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_result* res, wf_event* ev,
-                       u64 ev_cap, bool batched) {
+                       u64 ev_cap, bool batched, bool want_psp = false, u64* psp = nullptr, u64 psp_cap = 0) {
   if (!e || !r || !res || n == 0) return WF_E_ARG;
   if (e->nb != 1 || e->bases[0] != 2) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
@@ -990,7 +1056,8 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   const u64 want_ev = std::max<u64>(ev_cap, 1 << 16);
   if ((rc = grow(e->d_ev, e->dev_ev_cap, want_ev)) != WF_OK) return rc;
   HIPCHK(hipMemsetAsync(e->d_acc, 0, n * sizeof(DevAcc), e->st));
-  HIPCHK(hipMemsetAsync(e->d_ctr, 0, 2 * sizeof(unsigned long long), e->st));
+  HIPCHK(hipMemsetAsync(e->d_ctr, 0, 3 * sizeof(unsigned long long), e->st));
+  if (want_psp && (rc = grow(e->d_psp, e->psp_cap, std::max<u64>(psp_cap, 1 << 16))) != WF_OK) return rc;
   HIPCHK(hipMemsetAsync(e->d_flags, 0, sizeof(u32), e->st));
   TestOut to;
   to.acc = e->d_acc;
@@ -1036,7 +1103,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
     if ((rc = run_batched(e, passes, to)) != WF_OK) return rc;
   } else {
     for (auto& ps : passes)
-      if ((rc = run_pieces(e, ps.first, ps.second, to)) != WF_OK) return rc;
+      if ((rc = run_pieces(e, ps.first, ps.second, to, want_psp)) != WF_OK) return rc;
   }
   // results
   std::vector<DevAcc> acc(n);
@@ -1073,18 +1140,25 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   }
   e->last.tested = tested;
   if (!batched) e->last.total_ms = e->last.sieve_ms + e->last.test_ms;
+  for (u32 i = 0; i < n; i++) res[i].npsp = acc[i].npsp;
+  if (want_psp) {
+    unsigned long long np = 0;
+    HIPCHK(hipMemcpy(&np, e->d_ctr + 2, sizeof np, hipMemcpyDeviceToHost));
+    std::vector<u64> lst(std::min<u64>(np, e->psp_cap));
+    if (!lst.empty()) HIPCHK(hipMemcpy(lst.data(), e->d_psp, lst.size() * sizeof(u64), hipMemcpyDeviceToHost));
+    std::sort(lst.begin(), lst.end());
+    for (u64 k = 0; k < lst.size() && k < psp_cap; k++) psp[k] = lst[k];
+    if (np > e->psp_cap) return WF_E_OVERFLOW;
+  }
   if (nev > e->dev_ev_cap) return WF_E_OVERFLOW;
   return WF_OK;
 }
 
 extern "C" int wf_scan(wf_engine* e, u64 lo, u64 hi, int64_t near_T, wf_result* out, wf_event* ev, u64 ev_cap,
                        int want_psp, u64* psp, u64 psp_cap) {
-  if (!e || !out || (ev_cap && !ev)) return WF_E_ARG;
-  if (want_psp) return WF_E_UNSUPPORTED;
-  (void)psp;
-  (void)psp_cap;
+  if (!e || !out || (ev_cap && !ev) || (want_psp && psp_cap && !psp)) return WF_E_ARG;
   wf_range r = {lo, hi};
-  return scan_ranges(e, &r, 1, near_T, out, ev, ev_cap, false);
+  return scan_ranges(e, &r, 1, near_T, out, ev, ev_cap, false, want_psp != 0, psp, psp_cap);
 }
 
 extern "C" int wf_scan_batch(wf_engine* e, const wf_range* r, u32 n, int64_t near_T, wf_result* per_range,
@@ -1230,19 +1304,193 @@ extern "C" int wf_last_timing(wf_engine* e, wf_timing* t) {
   return WF_OK;
 }
 
+// ---------------------------------------------------------------------------------
+// serving: a device-resident request loop with continuous batching
+// ---------------------------------------------------------------------------------
+//
+// Admitted requests own a slot (a row of per-request accumulators in HBM).  The server
+// thread runs steps: each step takes the next piece of every active request (at least
+// min_piece numbers, at most an even share of the step budget), sieves and tests all of
+// them in the same kernel launches (range id = slot), and retires the requests whose last
+// piece is done.  Requests that arrive while a step runs join the next step.  The step
+// budget follows from the HBM budget and the bytes a pass needs per number.
+
+struct ServeReq {
+  u64 lo, hi, next;   // hi == 0: 2^64; next = first number not yet scheduled
+  u32 slot;
+  bool done;
+  wf_result res;
+};
+
+struct Server {
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  bool stop = false;
+  int err = WF_OK;
+  u32 slots = 0, active = 0;
+  u64 step_numbers = 0, min_piece = 0;
+  std::vector<u64> slot_ticket;   // 0 = free
+  std::unordered_map<u64, ServeReq> reqs;
+  u64 next_ticket = 1;
+  DevAcc* d_acc = nullptr;        // per slot
+  u64 steps = 0;
+};
+
+static void serve_loop(wf_engine* e) {
+  Server* S = e->srv;
+  (void)hipSetDevice(e->dev);
+  for (;;) {
+    std::vector<std::pair<u64, u64>> sched;     // (ticket, new next)
+    std::vector<Piece> all;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv.wait(lk, [&] { return S->stop || S->active > 0; });
+      if (S->stop) return;
+      const u64 share = std::max<u64>(S->min_piece, S->step_numbers / std::max<u32>(S->active, 1));
+      for (u32 k = 0; k < S->slots; k++) {
+        const u64 t = S->slot_ticket[k];
+        if (!t) continue;
+        ServeReq& q = S->reqs[t];
+        if (q.done) continue;
+        const double rem = span_d(q.next, q.hi);
+        const bool last = rem <= (double)share;
+        const u64 end = last ? q.hi : q.next + share;
+        split_classes(e, q.next, end, k, all);
+        sched.push_back({t, end});
+      }
+    }
+    int rc = WF_OK;
+    u64 maxtop = 0;
+    for (const Piece& p : all) maxtop = std::max<u64>(maxtop, p.hi ? p.hi - 1 : 0xFFFFFFFFFFFFFFFFull);
+    rc = ensure_sieving_primes(e, std::max<u64>(isqrt_u64(maxtop) + 1, 64));
+    TestOut to;
+    to.acc = S->d_acc;
+    to.ev = e->d_ev;
+    to.ev_cap = e->dev_ev_cap;
+    to.nev = e->d_ctr + 1;
+    to.near_T = 0;
+    for (int cls = A_F64; cls <= A_FULL && rc == WF_OK; cls++) {
+      std::vector<Piece> ps;
+      for (const Piece& p : all)
+        if (p.arith == cls) ps.push_back(p);
+      if (!ps.empty()) rc = run_pieces(e, ps, cls, to);
+    }
+    std::vector<DevAcc> acc(S->slots);
+    if (rc == WF_OK && hipMemcpy(acc.data(), S->d_acc, S->slots * sizeof(DevAcc), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = WF_E_DEVICE;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->steps++;
+    if (rc != WF_OK) {
+      S->err = rc;
+      for (auto& kv : S->reqs) kv.second.done = true;
+      S->cv.notify_all();
+      continue;
+    }
+    for (auto& sc : sched) {
+      ServeReq& q = S->reqs[sc.first];
+      q.next = sc.second;
+      if (q.next != q.hi) continue;
+      // finished: totals of its slot, plus the untested prime 2
+      const DevAcc& a = acc[q.slot];
+      memset(&q.res, 0, sizeof q.res);
+      q.res.primes = a.primes + ((q.lo <= 2 && (q.hi == 0 || q.hi > 2)) ? 1 : 0);
+      q.res.tests = a.tests;
+      q.res.near = a.near;
+      q.res.flt = a.flt;
+      q.res.cks[0] = a.cks;
+      q.res.sumq[0] = a.sumq;
+      q.res.nev = a.nev;
+      q.done = true;
+      (void)hipMemset(S->d_acc + q.slot, 0, sizeof(DevAcc));
+      S->slot_ticket[q.slot] = 0;
+      S->active--;
+    }
+    S->cv.notify_all();
+  }
+}
+
 extern "C" int wf_serve_start(wf_engine* e, u32 slots, u64 hbm_budget) {
-  (void)e; (void)slots; (void)hbm_budget;
-  return WF_E_UNSUPPORTED;
+  if (!e || slots == 0 || slots > 65536 || e->srv) return WF_E_ARG;
+  if (e->nb != 1 || e->bases[0] != 2) return WF_E_UNSUPPORTED;
+  HIPCHK(hipSetDevice(e->dev));
+  Server* S = new Server();
+  S->slots = slots;
+  S->slot_ticket.assign(slots, 0);
+  // bytes of pass buffers per number: prime regions + buckets (large primes up to 2^32)
+  const double SB = e->cfg.seg_bytes;
+  const double pcap = (double)prime_bound(30.0 * SB);
+  const double bcap = 8.0 * SB * 0.70 + 10.0 * std::sqrt(8.0 * SB * 0.70) + 1024.0;
+  const double per_number = (pcap + bcap) * 4.0 / (30.0 * SB);
+  if (!hbm_budget) hbm_budget = 16ull << 30;
+  S->step_numbers = (u64)std::min(2.0e10, (double)hbm_budget / per_number);
+  S->min_piece = std::max<u64>(30ull * e->cfg.seg_bytes, std::min<u64>(S->step_numbers / 64, 200000000ull));
+  if (S->step_numbers < S->min_piece) { delete S; return WF_E_ARG; }
+  if (hipMalloc(&S->d_acc, slots * sizeof(DevAcc)) != hipSuccess) { delete S; return WF_E_NOMEM; }
+  int rc;
+  if ((rc = grow(e->d_ev, e->dev_ev_cap, 1 << 16)) != WF_OK || hipMemset(S->d_acc, 0, slots * sizeof(DevAcc)) != hipSuccess ||
+      hipMemset(e->d_flags, 0, sizeof(u32)) != hipSuccess || hipMemset(e->d_ctr, 0, 3 * sizeof(unsigned long long)) != hipSuccess) {
+    (void)hipFree(S->d_acc);
+    delete S;
+    return rc != WF_OK ? rc : WF_E_DEVICE;
+  }
+  e->srv = S;
+  S->th = std::thread(serve_loop, e);
+  return WF_OK;
 }
+
 extern "C" int wf_serve_submit(wf_engine* e, u64 lo, u64 hi, u64* ticket) {
-  (void)e; (void)lo; (void)hi; (void)ticket;
-  return WF_E_UNSUPPORTED;
+  if (!e || !e->srv || !ticket || (hi != 0 && hi <= lo)) return WF_E_ARG;
+  Server* S = e->srv;
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->err) return S->err;
+  // admission: a free slot, and the step budget still gives every request min_piece
+  if (S->active >= S->slots || (u64)(S->active + 1) * S->min_piece > S->step_numbers) return WF_E_BUSY;
+  u32 k = 0;
+  while (S->slot_ticket[k]) k++;
+  const u64 t = S->next_ticket++;
+  ServeReq q;
+  q.lo = lo;
+  q.hi = hi;
+  q.next = lo;
+  q.slot = k;
+  q.done = false;
+  memset(&q.res, 0, sizeof q.res);
+  S->reqs[t] = q;
+  S->slot_ticket[k] = t;
+  S->active++;
+  *ticket = t;
+  S->cv.notify_all();
+  return WF_OK;
 }
+
 extern "C" int wf_serve_poll(wf_engine* e, u64 ticket, wf_result* out, int* done) {
-  (void)e; (void)ticket; (void)out; (void)done;
-  return WF_E_UNSUPPORTED;
+  if (!e || !e->srv || !out || !done) return WF_E_ARG;
+  Server* S = e->srv;
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(ticket);
+  if (it == S->reqs.end()) return WF_E_ARG;
+  if (S->err) return S->err;
+  *done = it->second.done ? 1 : 0;
+  if (*done) {
+    *out = it->second.res;
+    S->reqs.erase(it);
+  }
+  return WF_OK;
 }
+
 extern "C" int wf_serve_stop(wf_engine* e) {
-  (void)e;
-  return WF_E_UNSUPPORTED;
+  if (!e || !e->srv) return WF_E_ARG;
+  Server* S = e->srv;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv.notify_all();
+  if (S->th.joinable()) S->th.join();
+  (void)hipFree(S->d_acc);
+  const int err = S->err;
+  delete S;
+  e->srv = nullptr;
+  return err;
 }

@@ -180,6 +180,28 @@ class Engine:
         return {"seg_bytes": c.seg_bytes, "sieve_block": c.sieve_block, "bucket_min": c.bucket_min,
                 "sieve_ms_per_1e9": ms.value}
 
+    # serving (continuous batching inside the library; see wf.h)
+    def serve_start(self, slots=64, hbm_budget=0):
+        _check(self._L.wf_serve_start(self._h, slots, hbm_budget), "wf_serve_start")
+
+    def serve_submit(self, lo, hi):
+        """Ticket, or None when the server is full (WF_E_BUSY: retry later)."""
+        t = u64()
+        rc = self._L.wf_serve_submit(self._h, lo, _hi(hi), ctypes.byref(t))
+        if rc == -6:
+            return None
+        _check(rc, "wf_serve_submit")
+        return t.value
+
+    def serve_poll(self, ticket):
+        """Result dict once the request is done, else None."""
+        res, done = Result(), ctypes.c_int()
+        _check(self._L.wf_serve_poll(self._h, ticket, ctypes.byref(res), ctypes.byref(done)), "wf_serve_poll")
+        return result_dict(res) if done.value else None
+
+    def serve_stop(self):
+        _check(self._L.wf_serve_stop(self._h), "wf_serve_stop")
+
     def last_timing(self):
         t = Timing()
         _check(self._L.wf_last_timing(self._h, ctypes.byref(t)), "wf_last_timing")

@@ -29,4 +29,7 @@ void emu_mont(int L, int full, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo
   *hi = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
 }
 
+
+int emu_sprp2(uint64_t n) { return sprp2_64(n) ? 1 : 0; }
+
 }

@@ -23,6 +23,7 @@ def emu():
     P = ctypes.POINTER(ctypes.c_uint64)
     L.emu_fp64.argtypes = [ctypes.c_uint64, P, P]
     L.emu_mont.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, P, P]
+    L.emu_sprp2.argtypes = [ctypes.c_uint64]
     return L
 
 
@@ -66,3 +67,29 @@ def test_montgomery(emu, L, full, lo, hi):
         assert call(emu.emu_mont, L, full, p, eb) == pow(2, p - 1, p * p), (L, p)
         # a longer wave-wide exponent length only adds skipped leading bits
         assert call(emu.emu_mont, L, full, p, min(64, eb + 3)) == pow(2, p - 1, p * p), (L, p)
+
+
+def strong2(n):
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    x = pow(2, d, n)
+    if x in (1, n - 1):
+        return True
+    for _ in range(s - 1):
+        x = x * x % n
+        if x == n - 1:
+            return True
+    return False
+
+
+def test_sprp2_64(emu):
+    ns = list(range(3, 20001, 2)) + [2047, 3277, 4033, 4681, 8321, (1 << 64) - 59, (1 << 64) - 1, (1 << 63) + 1,
+                                     3825123056546413051, 318665857834031151167461 % (1 << 64) | 1]
+    rng = random.Random(3)
+    ns += [rng.getrandbits(64) | 1 for _ in range(3000)] + [rng.getrandbits(40) | 1 for _ in range(1000)]
+    for n in ns:
+        if n < 3:
+            continue
+        assert bool(emu.emu_sprp2(n)) == strong2(n), n

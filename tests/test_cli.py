@@ -1,0 +1,167 @@
+"""The wf host program: command line, number grammar, checkpoint validation and output
+formats of A001220/wieferich.c (CPU tests need no GPU: they stop before the engine opens;
+the gpu-marked tests scan for real)."""
+import os
+import re
+import signal
+import subprocess
+import sys
+import time
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WF = os.path.join(ROOT, "mathpub_amd", "wf")
+pytestmark_cpu = pytest.mark.skipif(not os.path.exists(WF), reason="wf not built")
+
+
+def run(*args, timeout=600):
+    p = subprocess.run([WF] + [str(a) for a in args], capture_output=True, text=True, timeout=timeout)
+    return p.returncode, p.stdout, p.stderr
+
+
+@pytestmark_cpu
+def test_usage_exit_2():
+    assert run()[0] == 2
+    assert run("frobnicate")[0] == 2
+    assert run("scan")[0] == 2
+    assert run("scan", "1", "2", "3")[0] == 2
+    assert run("scan", "10", "-z")[0] == 2
+
+
+@pytestmark_cpu
+@pytest.mark.parametrize("arg,msg", [("1e-5", "bad number"), ("e5", "bad number"), ("1.5e0", "bad number"),
+                                     ("2^65", "bad number"), ("5-7", "negative result"), ("1-2-3", "bad number"),
+                                     ("18446744073709551616+1", "overflow")])
+def test_number_grammar_errors(arg, msg):
+    rc, out, err = run("scan", arg, "2^70")
+    rc2, out2, err2 = run("scan", "0", arg)
+    assert rc == 1 or rc2 == 1
+    assert msg in err + err2
+
+
+@pytestmark_cpu
+def test_range_errors():
+    assert "END must be larger than START" in run("scan", "10", "5")[2]
+    assert "START must be < 2^64" in run("scan", "2^64", "2^64")[2]
+    rc, out, err = run("scan", "2^64-10", "2^64+0", "-c", "7")
+    assert rc == 1 and "END must be <= 2^64" in err          # rounded up past 2^64
+    assert "P must be odd" in run("check", "4")[2]
+    assert "bad base" in run("scan", "10", "-b", "1")[2]
+
+
+def write_ckpt(path, **kw):
+    f = {"bases": "2", "start": 1000, "chunk": 100, "end": 2000, "frontier": 1, "primes": 5, "tests": 5,
+         "near": 0, "flt": 0}
+    f.update(kw)
+    lines = ["wieferich checkpoint 1"]
+    for k in ("bases", "start", "chunk", "end", "frontier", "primes", "tests", "near", "flt"):
+        if f[k] is not None:
+            lines.append("%s %s" % (k, f[k]))
+    if kw.get("cks", True):
+        lines.append("cks 2 0123456789abcdef")
+    if kw.get("terminated", True):
+        lines.append("end")
+    open(path, "w").write("\n".join(lines) + "\n")
+
+
+@pytestmark_cpu
+@pytest.mark.parametrize("kw,msg", [({"terminated": False}, "is truncated"), ({"cks": False}, "has no checksum"),
+                                    ({"start": 999}, "was written for start=999"),
+                                    ({"chunk": 50}, "chunk=50"), ({"bases": "3"}, "was written for bases 3"),
+                                    ({"frontier": 99}, "is past the new END")])
+def test_checkpoint_validation(tmp_path, kw, msg):
+    ck = tmp_path / "st"
+    write_ckpt(ck, **kw)
+    rc, out, err = run("scan", "1000", "2000", "-c", "100", "-S", ck)
+    assert rc == 1 and msg in err, err
+
+
+@pytestmark_cpu
+def test_foreign_file_rejected(tmp_path):
+    ck = tmp_path / "st"
+    ck.write_text("hello\n")
+    rc, out, err = run("scan", "1000", "2000", "-c", "100", "-S", ck)
+    assert rc == 1 and "not a wieferich checkpoint" in err
+
+
+# ---- on the GPU ----
+
+def parse_summary(out):
+    m = re.search(r"SUMMARY \[(\d+), (\d+)\) primes (\d+) tests (\d+) flt_errors (\d+)", out)
+    cks = re.search(r"base   2 A001220  cks ([0-9a-f]{16})", out)
+    return m.groups(), cks.group(1)
+
+
+@pytest.mark.gpu
+def test_scan_matches_oracle_and_log_format(tmp_path):
+    import oracle
+    log = tmp_path / "log"
+    rc, out, err = run("scan", "1e15", "1e15+3e8", "-c", "1e8", "-n", "1000", "-L", log, "-q", "-Q")
+    assert rc == 0, err
+    (lo, hi, primes, tests, flt), cks = parse_summary(out)
+    o = oracle.scan(10 ** 15, 10 ** 15 + 3 * 10 ** 8, bases=[2], near_T=1000)
+    assert int(primes) == o["primes"] and cks == "%016x" % o["cks"][2] and int(flt) == 0
+    assert "sumq base 2 %016x" % o["sumq"][2] in out
+    lines = log.read_text().splitlines()
+    assert len(lines) == 3
+    tot = 0
+    for k, ln in enumerate(lines):
+        a = 10 ** 15 + k * 10 ** 8
+        oc = oracle.scan(a, a + 10 ** 8, bases=[2])
+        assert ln == "C %d %d %d 2:%016x" % (a, a + 10 ** 8, oc["primes"], oc["cks"][2])
+        tot += oc["primes"]
+    assert tot == o["primes"]
+    near = [l for l in out.splitlines() if l.startswith("NEAR")]
+    assert len(near) == o["near"]
+
+
+@pytest.mark.gpu
+def test_found_and_confirmed_lines():
+    rc, out, err = run("scan", "0", "1e5", "-c", "2e4", "-q")
+    assert rc == 0
+    assert "FOUND base 2 p 1093 (A001220)" in out and "FOUND base 2 p 3511 (A001220)" in out
+    assert "CONFIRMED base 2 p 1093 (A001220) (all primes in [0, 20000) tested)" in out
+    assert re.search(r"base   2 A001220  cks [0-9a-f]{16}  solutions: 1093 3511", out)
+
+
+@pytest.mark.gpu
+def test_resume_equals_straight_run(tmp_path):
+    ck = tmp_path / "st"
+    straight = run("scan", "1e12", "1e12+4e8", "-c", "5e7", "-q")[1]
+    r1 = run("scan", "1e12", "1e12+2e8", "-c", "5e7", "-q", "-S", ck)
+    assert r1[0] == 0 and ck.exists()
+    r2 = run("scan", "1e12", "1e12+4e8", "-c", "5e7", "-q", "-S", ck)      # END may change on resume
+    assert r2[0] == 0
+    assert parse_summary(r2[1]) == parse_summary(straight)
+
+
+@pytest.mark.gpu
+def test_sigint_saves_checkpoint_then_resumes(tmp_path):
+    ck = tmp_path / "st"
+    straight = run("scan", "1e15", "1e15+3e10", "-c", "1e9", "-q")[1]
+    p = subprocess.Popen([WF, "scan", "1e15", "1e15+3e10", "-c", "1e9", "-S", str(ck)], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True)
+    time.sleep(3.0)
+    p.send_signal(signal.SIGINT)
+    out, err = p.communicate(timeout=300)
+    assert p.returncode == 0 and ck.exists()
+    assert "stopped at position" in err or "SUMMARY [1000000000000000, 1000030000000000)" in out
+    r2 = run("scan", "1e15", "1e15+3e10", "-c", "1e9", "-q", "-S", ck)
+    assert parse_summary(r2[1]) == parse_summary(straight)
+
+
+@pytest.mark.gpu
+def test_check_command():
+    rc, out, err = run("check", "1093")
+    assert rc == 0 and "q = 0  A = 0  <-- WIEFERICH  [reference agrees]" in out
+    rc, out, err = run("check", "18446744073709551557")
+    assert rc == 0 and "(prime)" in out and "agrees" in out
+    rc, out, err = run("check", "2047")
+    assert rc == 0 and "NOT prime" in out
+
+
+@pytest.mark.gpu
+def test_selftest_small():
+    rc, out, err = run("selftest", "1e8", timeout=900)
+    assert rc == 0 and "selftest passed" in err, err

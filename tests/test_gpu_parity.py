@@ -63,8 +63,10 @@ def test_quotients_vs_pow(eng):
 def test_golden_windows(eng, w):
     g = WINS[w]
     say("golden window", g["lo"], g["hi"])
-    r = eng.scan(g["lo"], g["hi"], near_T=g["near_T"])
+    r = eng.scan(g["lo"], g["hi"], near_T=g["near_T"], psp=g["psp"] is not None)
     assert r["primes"] == g["primes"]
+    if g["psp"] is not None:
+        assert r["psp"] == g["psp"] and r["npsp"] == len(g["psp"])
     assert "%016x" % r["cks"] == g["cks2"]
     assert "%016x" % r["sumq"] == g["sumq2"]
     assert r["flt"] == 0
@@ -105,3 +107,63 @@ def test_batch_equals_single(eng):
     for (lo, hi), r in zip(ranges, res):
         s = eng.scan(lo, hi)
         assert (r["primes"], r["tests"], r["cks"], r["sumq"]) == (s["primes"], s["tests"], s["cks"], s["sumq"]), (lo, hi)
+
+
+@pytest.mark.parametrize("lo,hi", [(10 ** 12, 10 ** 12 + 3 * 10 ** 6), ((1 << 64) - 4 * 10 ** 6, 1 << 64),
+                                   (1, 10 ** 7)])
+def test_psp_vs_oracle(eng, lo, hi):
+    say("psp", lo, hi)
+    r = eng.scan(lo, hi, psp=True)
+    o = oracle.scan(lo, hi, bases=[2], psp=True)
+    assert r["psp"] == o["psp"] and r["npsp"] == o["npsp"]
+    assert r["primes"] == o["primes"] and r["cks"] == o["cks"][2]
+
+
+def test_autotune_keeps_results(eng):
+    lo, hi = 10 ** 15, 10 ** 15 + 5 * 10 ** 8
+    before = eng.scan(lo, hi)
+    cfg = eng.autotune(lo, hi)
+    say("autotune", cfg)
+    assert cfg["seg_bytes"] in (16384, 32768, 65536)
+    after = eng.scan(lo, hi)
+    per, ev = eng.scan_batch([(lo, hi)])
+    for x in (after, per[0]):
+        assert (x["primes"], x["cks"], x["sumq"]) == (before["primes"], before["cks"], before["sumq"])
+
+
+def test_serving_same_alone_and_in_mix():
+    """Continuous batching: every request's result equals the same range scanned alone."""
+    import time
+    rng = random.Random(11)
+    ranges = []
+    for _ in range(48):
+        lo = rng.randrange(10 ** 9, 10 ** 18)
+        ranges.append((lo, lo + int(10 ** rng.uniform(5, 8.5))))
+    ranges += [(0, 10 ** 6), ((1 << 50) - 10 ** 6, (1 << 50) + 10 ** 6), ((1 << 64) - 10 ** 6, 1 << 64)]
+    with wf.Engine() as alone:
+        want = [alone.scan(lo, hi) for lo, hi in ranges]
+    with wf.Engine() as eng:
+        eng.serve_start(slots=16, hbm_budget=8 << 30)
+        pending = list(enumerate(ranges))
+        live, got = {}, {}
+        t0 = time.time()
+        while (pending or live) and time.time() - t0 < 240:
+            while pending:
+                i, (lo, hi) = pending[0]
+                t = eng.serve_submit(lo, hi)
+                if t is None:
+                    break
+                live[t] = i
+                pending.pop(0)
+            for t in list(live):
+                r = eng.serve_poll(t)
+                if r is not None:
+                    got[live.pop(t)] = r
+            time.sleep(0.002)
+        eng.serve_stop()
+    say("served", len(got), "of", len(ranges))
+    assert len(got) == len(ranges)
+    for i, w in enumerate(want):
+        g = got[i]
+        assert (g["primes"], g["tests"], g["cks"], g["sumq"], g["flt"], g["nev"]) == \
+            (w["primes"], w["tests"], w["cks"], w["sumq"], w["flt"], w["nev"]), ranges[i]
