@@ -32,6 +32,12 @@ def build_lib():
     os.replace(tmp, LIB)
 
 
+def build_probe():
+    src = os.path.join(PKG, "csrc", "wf_probe.hip")
+    flags = [f for f in HIP_FLAGS if f not in ("-shared", "-fPIC")]
+    run([HIPCC] + flags + ["-o", os.path.join(PKG, "wf_probe"), src])
+
+
 def build_host():
     src = os.path.join(PKG, "host", "wf.c")
     if not os.path.exists(src):
@@ -43,6 +49,7 @@ def build_host():
 def main():
     build_lib()
     build_host()
+    build_probe()
 
 
 if __name__ == "__main__":

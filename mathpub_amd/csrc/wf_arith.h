@@ -54,28 +54,38 @@ __device__ __forceinline__ u64 inv64(u64 p) {
 // ---------------------------------------------------------------------------------
 //
 // Invariants (u = 2^-53, p < 2^50, all values integers held exactly):
-//   |a| <= 0.667 p (0.5 p right after a window step), |b| <= 1.9 p.
-// Squaring, x^2 = a^2 + 2ab p (mod p^2):
-//   a^2 = k1 p + a'      k1 = rint(fl(a*a) * ip): |k1 - a^2/p| <= 0.5 + 0.375 alpha^2,
-//                       so alpha' = 0.5 + 0.375 alpha^2, fixed point 0.667;
-//   ab  = (k2 p + e)/2   k2 = rint(fl(a*b) * 2ip), e/2 = fma(-k2, p/2, h) + l exact
-//                       (a multiple of 1/2 below 2^50 in magnitude);
-//   b'  = k1 + e        |b'| <= alpha^2 p + 1 + (0.5 + 0.75 alpha beta) p, fixed point 1.9 p.
-// Every intermediate stays below 2^52 in magnitude, so each FMA result is exact.
+//   |a| <= 0.59 p (0.5 p right after a window step), |b| <= 1.9 p.
+// Squaring, x^2 = a^2 + 2ab p (mod p^2), quotients rounded once inside an FMA:
+//   a^2 = k1 p + a'      k1 = round(fl(a*a) / p): |k1 - a^2/p| <= 0.5 + 0.25 alpha^2,
+//                       so alpha' = 0.5 + 0.25 alpha^2 (fixed point 0.59);
+//   ab  = k2 p + r       k2 = round(fl(a*b) / p): |r| <= (0.5 + 0.25 alpha beta) p;
+//   b'  = k1 + 2r        |b'| <= alpha^2 p + 1 + (1 + 0.5 alpha beta) p, so
+//                       beta' = 1.34 + 0.29 beta (fixed point 1.9).
+// All quotients stay below 2^51 in magnitude (round_mul's range) and every remainder below
+// 2^52, so each FMA result is exact.
 struct F64Mod {
-  double p, ip, ip2, ph;   // p, 1/p, 2/p, p/2
+  double p, ip;   // p, 1/p
 };
+
+// Round-to-nearest integer of x*y + 0 for |x*y| < 2^51: adding 1.5 * 2^52 inside the FMA
+// leaves a sum in [2^52, 2^53), whose unit in the last place is 1, so the single rounding
+// of the FMA is the rounding to an integer.  (v_rndne_f64 issues at half rate on gfx950;
+// FMA and add at full rate.)
+#define WF_RND_MAGIC 6755399441055744.0
+__device__ __forceinline__ double round_mul(double x, double y) {
+  return __builtin_fma(x, y, WF_RND_MAGIC) - WF_RND_MAGIC;
+}
 
 __device__ __forceinline__ void f64_sqr(double& a, double& b, const F64Mod& m) {
   const double h1 = a * a;
   const double l1 = __builtin_fma(a, a, -h1);
-  const double k1 = __builtin_rint(h1 * m.ip);
-  const double a1 = __builtin_fma(-k1, m.p, h1) + l1;
+  const double k1 = round_mul(h1, m.ip);
+  const double a1 = __builtin_fma(-k1, m.p, h1) + l1;      // a^2 - k1 p
   const double h2 = a * b;
   const double l2 = __builtin_fma(a, b, -h2);
-  const double k2 = __builtin_rint(h2 * m.ip2);
-  const double e2 = __builtin_fma(-k2, m.ph, h2) + l2;  // (2ab - k2 p) / 2
-  b = __builtin_fma(2.0, e2, k1);
+  const double k2 = round_mul(h2, m.ip);
+  const double r2 = __builtin_fma(-k2, m.p, h2) + l2;      // ab - k2 p
+  b = __builtin_fma(2.0, r2, k1);                          // k1 + 2ab - 2 k2 p
   a = a1;
 }
 
@@ -83,12 +93,12 @@ __device__ __forceinline__ void f64_sqr(double& a, double& b, const F64Mod& m) {
 // reduced by an exact FMA remainder, and the carry ka moves from a into b.
 __device__ __forceinline__ void f64_mul_pow2(double& a, double& b, int d, const F64Mod& m) {
   const double A = __builtin_ldexp(a, d);
-  const double ka = __builtin_rint(A * m.ip);
+  const double ka = round_mul(A, m.ip);
   a = __builtin_fma(-ka, m.p, A);
   const double B = __builtin_ldexp(b, d);
-  const double kb = __builtin_rint(B * m.ip);
+  const double kb = round_mul(B, m.ip);
   const double b1 = __builtin_fma(-kb, m.p, B) + ka;
-  const double kc = __builtin_rint(b1 * m.ip);
+  const double kc = round_mul(b1, m.ip);
   b = __builtin_fma(-kc, m.p, b1);
 }
 
@@ -99,8 +109,6 @@ __device__ __forceinline__ void fp64_pow2(u64 p, u64 e, int ebits, double& ra, d
   F64Mod m;
   m.p = (double)p;
   m.ip = 1.0 / m.p;
-  m.ip2 = 2.0 * m.ip;
-  m.ph = 0.5 * m.p;
   const int nw = (ebits + W - 1) / W;
   int sh = (nw - 1) * W;
   double a = 1.0, b = 0.0;

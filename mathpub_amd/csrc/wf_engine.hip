@@ -103,6 +103,8 @@ __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c) {
 // (byte << 3 | class) per hit in the bucket of the segment it lands in.
 
 #define RUN_MAX 4096      // segments per run (LDS cursors of k_bucket_fill)
+#define PRE_PERIOD 323323u  // 7 * 11 * 13 * 17 * 19: period of the presieve pattern in wheel bytes
+#define PRE_COUNT 5u        // the presieve primes are the first five sieving primes
 #define BF_PRIMES 2048    // large primes per k_bucket_fill workgroup
 
 struct SegDesc {
@@ -207,7 +209,8 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
                                                 u32 bcap,
                                                 u32* __restrict__ out, u32* __restrict__ cnt, u32 pcap,
                                                 DevAcc* __restrict__ acc, u32* __restrict__ flags,
-                                                u32* __restrict__ bitmap, u32 bmw) {
+                                                u32* __restrict__ bitmap, u32 bmw,
+                                                const u32* __restrict__ pre, u32 pre_words) {
   extern __shared__ u32 seg[];
   __shared__ u32 wtot[16];
   const int lane = threadIdx.x & 63;
@@ -217,11 +220,18 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
     const SegDesc d = segs[s];
     const u32 nwords = d.nbytes >> 2;
     const u64 span = 30ull * d.nbytes;
-    for (u32 w = threadIdx.x; w < nwords; w += blockDim.x) seg[w] = 0;
+    // start from the pattern of the presieve primes 7, 11, 13, 17, 19 (period 323323 bytes)
+    {
+      const u32 ph = (u32)((d.base / 30) % PRE_PERIOD);
+      const u32* pw = pre + (u64)(ph & 3) * pre_words + (ph >> 2);
+      for (u32 w = threadIdx.x; w < nwords; w += blockDim.x) seg[w] = pw[w];
+    }
+    __syncthreads();
+    if (d.base == 0 && threadIdx.x == 0) seg[0] &= ~0x3Eu;   // 7, 11, 13, 17, 19 themselves are prime
     __syncthreads();
     const u32 nsmall = min(d.nq, q1idx);
     // tier A: one wavefront per prime, lanes stride over the marks of each class
-    for (u32 i = wave; i < min(tierA, nsmall); i += nwaves) {
+    for (u32 i = PRE_COUNT + wave; i < min(tierA, nsmall); i += nwaves) {
       const u32 q = sp[i];
       u64 o;
       if (!first_offset(d.base, q, spinv[i], span, o)) continue;
@@ -537,6 +547,8 @@ struct wf_engine {
   u64 run_cap = 0;
   BufSet bs[2];
   u32* d_flags = nullptr;
+  u32* d_pre = nullptr;          // presieve pattern, 4 copies shifted by 0..3 bytes
+  u32 pre_words = 0;
   hipStream_t st2 = nullptr;     // test kernels of the batched pipeline
   hipEvent_t gev[5] = {};        // graph capture: fork, sieved[2], tested[2]
   DevAcc* d_acc = nullptr;
@@ -699,6 +711,35 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
     wf_close(e);
     return WF_E_DEVICE;
   }
+  // presieve pattern: byte J has bit c set iff 30 J + R30[c] is divisible by 7, 11, 13, 17 or 19;
+  // copy k (k = 0..3) holds the words starting at byte 4 i + k, long enough for any phase
+  // plus the largest segment
+  {
+    const u32 ext = PRE_PERIOD + 65536 + 8;
+    std::vector<unsigned char> pat(ext);
+    for (u32 j = 0; j < ext; j++) {
+      const u64 J = j % PRE_PERIOD;
+      unsigned char b = 0;
+      for (int c = 0; c < 8; c++) {
+        const u64 n = 30 * J + R30H[c];
+        if (n % 7 == 0 || n % 11 == 0 || n % 13 == 0 || n % 17 == 0 || n % 19 == 0) b |= (unsigned char)(1u << c);
+      }
+      pat[j] = b;
+    }
+    e->pre_words = (PRE_PERIOD + 65536) / 4 + 1;
+    std::vector<u32> w4(4 * (size_t)e->pre_words);
+    for (u32 k = 0; k < 4; k++)
+      for (u32 i = 0; i < e->pre_words; i++) {
+        const u32 o = 4 * i + k;
+        w4[(size_t)k * e->pre_words + i] = (u32)pat[o] | ((u32)pat[o + 1] << 8) | ((u32)pat[o + 2] << 16) |
+                                          ((u32)pat[o + 3] << 24);
+      }
+    if (hipMalloc(&e->d_pre, w4.size() * sizeof(u32)) != hipSuccess ||
+        hipMemcpy(e->d_pre, w4.data(), w4.size() * sizeof(u32), hipMemcpyHostToDevice) != hipSuccess) {
+      wf_close(e);
+      return WF_E_NOMEM;
+    }
+  }
   if (hipMalloc(&e->d_ctr, 8 * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&e->d_flags, 8 * sizeof(u32)) != hipSuccess) {
     wf_close(e);
@@ -726,6 +767,7 @@ extern "C" void wf_close(wf_engine* e) {
     (void)hipFree(e->bs[k].bitmap);
   }
   (void)hipFree(e->d_psp);
+  (void)hipFree(e->d_pre);
   (void)hipFree(e->d_flags);
   (void)hipFree(e->d_acc);
   (void)hipFree(e->d_ev);
@@ -883,7 +925,7 @@ static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, 
   const u32 sgrid = (u32)std::min<u64>(nseg, (u64)cus * 4);
   hipLaunchKernelGGL(k_sieve, dim3(sgrid), dim3(e->cfg.sieve_block), e->cfg.seg_bytes, st, dsegs, (u32)nseg, e->d_sp,
                      e->d_spinv, e->tierA, e->q1idx, B.bcnt, B.bdata, P.bcap, B.off, B.cnt, P.pcap, acc, e->d_flags,
-                     want_bitmap ? B.bitmap : nullptr, e->cfg.seg_bytes / 4);
+                     want_bitmap ? B.bitmap : nullptr, e->cfg.seg_bytes / 4, e->d_pre, e->pre_words);
   HIPCHK(hipGetLastError());
   return WF_OK;
 }
