@@ -102,6 +102,16 @@ __device__ __forceinline__ void f64_mul_pow2(double& a, double& b, int d, const 
   b = __builtin_fma(-kc, m.p, b1);
 }
 
+// a in [0, p), then b in [0, p): the canonical digits of r = a + b p.
+__device__ __forceinline__ void f64_normalise(double& a, double& b, const F64Mod& m) {
+  if (a < 0.0) { a += m.p; b -= 1.0; }
+  else if (a >= m.p) { a -= m.p; b += 1.0; }
+  const double kb = __builtin_floor(b * m.ip);
+  b = __builtin_fma(-kb, m.p, b);
+  if (b < 0.0) b += m.p;
+  else if (b >= m.p) b -= m.p;
+}
+
 // r = 2^e mod p^2 as (a, b) normalised to 0 <= a, b < p (r = a + b p).  ebits = the
 // number of bits to process (the wave's longest exponent), W = window width.
 template <int W>
@@ -118,15 +128,37 @@ __device__ __forceinline__ void fp64_pow2(u64 p, u64 e, int ebits, double& ra, d
     for (int s = 0; s < W; s++) f64_sqr(a, b, m);
     f64_mul_pow2(a, b, (int)((e >> sh) & ((1u << W) - 1)), m);
   }
-  // normalise: a in [0, p), then b in [0, p)
-  if (a < 0.0) { a += m.p; b -= 1.0; }
-  else if (a >= m.p) { a -= m.p; b += 1.0; }
-  const double kb = __builtin_floor(b * m.ip);
-  b = __builtin_fma(-kb, m.p, b);
-  if (b < 0.0) b += m.p;
-  else if (b >= m.p) b -= m.p;
+  f64_normalise(a, b, m);
   ra = a;
   rb = b;
+}
+
+// Two independent ladders per lane (two primes): the two instruction streams interleave,
+// which hides FP64 latency without more wavefronts.
+template <int W>
+__device__ __forceinline__ void fp64_pow2x2(u64 p0, u64 e0, u64 p1, u64 e1, int ebits, double& a0, double& b0,
+                                            double& a1, double& b1) {
+  F64Mod m0, m1;
+  m0.p = (double)p0;
+  m0.ip = 1.0 / m0.p;
+  m1.p = (double)p1;
+  m1.ip = 1.0 / m1.p;
+  const int nw = (ebits + W - 1) / W;
+  int sh = (nw - 1) * W;
+  a0 = 1.0; b0 = 0.0; a1 = 1.0; b1 = 0.0;
+  f64_mul_pow2(a0, b0, (int)(e0 >> sh), m0);
+  f64_mul_pow2(a1, b1, (int)(e1 >> sh), m1);
+  for (sh -= W; sh >= 0; sh -= W) {
+#pragma unroll
+    for (int s = 0; s < W; s++) {
+      f64_sqr(a0, b0, m0);
+      f64_sqr(a1, b1, m1);
+    }
+    f64_mul_pow2(a0, b0, (int)((e0 >> sh) & ((1u << W) - 1)), m0);
+    f64_mul_pow2(a1, b1, (int)((e1 >> sh) & ((1u << W) - 1)), m1);
+  }
+  f64_normalise(a0, b0, m0);
+  f64_normalise(a1, b1, m1);
 }
 
 // ---------------------------------------------------------------------------------

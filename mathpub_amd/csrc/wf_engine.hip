@@ -85,10 +85,15 @@ __constant__ unsigned char c_QI[30];
 
 // Byte index j (relative to the segment base) of the first mark of class c for a prime q
 // whose first multiple at or after max(base, q^2) lies at offset o (numbers) from base.
+// o < 2^32 at every call (o < q, or o < the span of a segment or fill run).  The
+// numerator is a multiple of 30, so below 30 * 2^32 the division is exact: halve, then
+// multiply by 15^-1 mod 2^32.  Larger starts lie beyond every segment and run.
 __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c) {
-  const u32 om = (u32)(o % 30);
-  const u32 m = c_M30[(qi * 30 + om) * 8 + c];
-  return (o + (u64)m * q - c_R30[c]) / 30;
+  const u32 o32 = (u32)o;
+  const u32 m = c_M30[(qi * 30 + o32 % 30u) * 8 + c];
+  const u64 v = (u64)m * q + o32 - c_R30[c];
+  if (v >= (30ull << 32)) return ~0ull;   // beyond any segment or run (q close to 2^32)
+  return (u64)((u32)(v >> 1) * 0xEEEEEEEFu);
 }
 
 // ---------------------------------------------------------------------------------
@@ -102,10 +107,11 @@ __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c) {
 // each such prime over a run of consecutive segments once and leaves one 4-byte record
 // (byte << 3 | class) per hit in the bucket of the segment it lands in.
 
-#define RUN_MAX 4096      // segments per run (LDS cursors of k_bucket_fill)
+#define RUN_MAX 1024      // segments per run (fill runs are cut shorter for smaller primes)
 #define PRE_PERIOD 323323u  // 7 * 11 * 13 * 17 * 19: period of the presieve pattern in wheel bytes
 #define PRE_COUNT 5u        // the presieve primes are the first five sieving primes
-#define BF_PRIMES 2048    // large primes per k_bucket_fill workgroup
+#define FILL_STAGE 8192   // records a fill workgroup stages in LDS (32 KiB)
+#define FILL_TARGET 4096  // expected records per fill work item (the host sizes items to it)
 
 struct SegDesc {
   u64 base;       // multiple of 30
@@ -119,6 +125,10 @@ struct SegDesc {
 
 // consecutive segments seg0 .. seg0 + nseg - 1 of one range piece, base = seg0's base
 struct RunDesc { u64 base; u32 seg0, nseg, nq, pad; };
+
+// one k_bucket_fill work item: sieving primes [i0, i1) over the nseg segments from seg0
+// (their numbers start at base)
+struct FillDesc { u64 base; u32 seg0, nseg, i0, i1; };
 
 struct DevAcc {     // per range; every field is a 64-bit sum
   unsigned long long primes, tests, near, flt, cks, sumq, nev, npsp;
@@ -139,52 +149,86 @@ __device__ __forceinline__ bool first_offset(u64 base, u32 q, u64 minv, u64 span
   return o < span;
 }
 
-// Large primes sp[q1idx .. run.nq) -> bucket records of the run's segments.  Two walks
-// over the hits: the first counts per segment in LDS and reserves room in each bucket
-// with one global atomic per (workgroup, segment), the second writes the records.
-__global__ void __launch_bounds__(256) k_bucket_fill(const RunDesc* __restrict__ runs, u32 npb,
+// Large primes -> bucket records of a run's segments.  A work item covers a slice of the
+// large primes over one run of <= RUN_MAX segments, sized by the host to about FILL_TARGET
+// expected records.  Walk 1 counts the hits per segment in LDS; one global atomic per
+// segment reserves room in its bucket; walk 2 stages the records in LDS grouped by
+// segment; the workgroup then copies each group to its bucket with consecutive lanes on
+// consecutive addresses.  Records that do not fit the stage go straight to the bucket.
+__global__ void __launch_bounds__(256) k_bucket_fill(const FillDesc* __restrict__ items,
                                                      const u32* __restrict__ sp, const u64* __restrict__ spinv,
-                                                     u32 q1idx, u32* __restrict__ bcnt, u32* __restrict__ bdata,
+                                                     u32* __restrict__ bcnt, u32* __restrict__ bdata,
                                                      u32 bcap, int sbshift, u32* __restrict__ flags) {
-  __shared__ u32 cur[RUN_MAX];
-  const RunDesc r = runs[blockIdx.x / npb];
-  const u32 i0 = q1idx + (blockIdx.x % npb) * BF_PRIMES;
-  if (i0 >= r.nq) return;
-  const u32 i1 = min(i0 + BF_PRIMES, r.nq);
+  __shared__ u32 hist[RUN_MAX], loff[RUN_MAX + 1], gbase[RUN_MAX];
+  __shared__ u32 stage[FILL_STAGE];
+  const FillDesc it = items[blockIdx.x];
+  const FillDesc& r = it;
   const u64 nbytes = (u64)r.nseg << sbshift;
   const u64 span = 30ull * nbytes;
   const u32 smask = (1u << sbshift) - 1;
-  for (u32 k = threadIdx.x; k < r.nseg; k += blockDim.x) cur[k] = 0;
+  for (u32 k = threadIdx.x; k < r.nseg; k += blockDim.x) hist[k] = 0;
   __syncthreads();
-  for (u32 i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+  // few primes with many hits each: one lane per (prime, class); else one lane per prime
+  const bool pairs = it.i1 - it.i0 < blockDim.x;
+  const u32 nwork = pairs ? (it.i1 - it.i0) * 8 : it.i1 - it.i0;
+  for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
+    const u32 i = it.i0 + (pairs ? w >> 3 : w);
     const u32 q = sp[i];
     u64 o;
     if (!first_offset(r.base, q, spinv[i], span, o)) continue;
     const int qi = c_QI[q % 30];
-    for (int c = 0; c < 8; c++)
-      for (u64 j = class_start(o, q, qi, c); j < nbytes; j += q) atomicAdd(&cur[j >> sbshift], 1u);
+    const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
+    for (int c = c0; c < c1; c++)
+      for (u64 j = class_start(o, q, qi, c); j < nbytes; j += q) atomicAdd(&hist[j >> sbshift], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 acc = 0;
+    for (u32 k = 0; k < r.nseg; k++) {
+      loff[k] = acc;
+      acc += hist[k];
+    }
+    loff[r.nseg] = acc;
   }
   __syncthreads();
   for (u32 k = threadIdx.x; k < r.nseg; k += blockDim.x) {
-    const u32 h = cur[k];
+    const u32 h = hist[k];
+    u32 b = 0;
     if (h) {
-      const u32 b = atomicAdd(&bcnt[r.seg0 + k], h);
+      b = atomicAdd(&bcnt[r.seg0 + k], h);
       if (b + h > bcap) atomicOr(flags, 1u);
-      cur[k] = b;
     }
+    gbase[k] = b;
+    hist[k] = loff[k];   // becomes the staging cursor
   }
   __syncthreads();
-  for (u32 i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+  for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
+    const u32 i = it.i0 + (pairs ? w >> 3 : w);
     const u32 q = sp[i];
     u64 o;
     if (!first_offset(r.base, q, spinv[i], span, o)) continue;
     const int qi = c_QI[q % 30];
-    for (int c = 0; c < 8; c++)
+    const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
+    for (int c = c0; c < c1; c++)
       for (u64 j = class_start(o, q, qi, c); j < nbytes; j += q) {
-        const u32 s = (u32)(j >> sbshift);
-        const u32 slot = atomicAdd(&cur[s], 1u);
-        if (slot < bcap) bdata[(u64)(r.seg0 + s) * bcap + slot] = (((u32)j & smask) << 3) | (u32)c;
+        const u32 sg = (u32)(j >> sbshift);
+        const u32 rec = (((u32)j & smask) << 3) | (u32)c;
+        const u32 slot = atomicAdd(&hist[sg], 1u);
+        if (slot < FILL_STAGE) {
+          stage[slot] = (sg << 19) | rec;
+        } else {
+          const u32 pos = gbase[sg] + (slot - loff[sg]);
+          if (pos < bcap) bdata[(u64)(r.seg0 + sg) * bcap + pos] = rec;
+        }
       }
+  }
+  __syncthreads();
+  const u32 nst = min(loff[r.nseg], (u32)FILL_STAGE);
+  for (u32 k = threadIdx.x; k < nst; k += blockDim.x) {
+    const u32 v = stage[k];
+    const u32 sg = v >> 19;
+    const u32 pos = gbase[sg] + (k - loff[sg]);
+    if (pos < bcap) bdata[(u64)(r.seg0 + sg) * bcap + pos] = v & 0x7FFFFu;
   }
 }
 
@@ -236,10 +280,11 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
       u64 o;
       if (!first_offset(d.base, q, spinv[i], span, o)) continue;
       const int qi = c_QI[q % 30];
+      const u32 jl = (u32)class_start(o, q, qi, lane & 7);     // lane c (< 8) holds class c's start
       for (int c = 0; c < 8; c++) {
-        const u64 j0 = class_start(o, q, qi, c);
-        for (u64 j = j0 + (u64)lane * q; j < d.nbytes; j += 64ull * q)
-          atomicOr(&seg[j >> 2], 1u << ((((u32)j & 3) << 3) | c));
+        const u32 j0 = __shfl(jl, c, 64);
+        for (u32 j = j0 + (u32)lane * q; j < d.nbytes; j += 64u * q)
+          atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c));
       }
     }
     // tier B: one lane per prime
@@ -250,8 +295,8 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
       const int qi = c_QI[q % 30];
 #pragma unroll
       for (int c = 0; c < 8; c++) {
-        for (u64 j = class_start(o, q, qi, c); j < d.nbytes; j += q)
-          atomicOr(&seg[j >> 2], 1u << ((((u32)j & 3) << 3) | c));
+        for (u32 j = (u32)class_start(o, q, qi, c); j < d.nbytes; j += q)
+          atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c));
       }
     }
     // large primes: this segment's bucket
@@ -325,6 +370,10 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
 
 #define TEST_TILE 1024   // primes per test work item (4 rounds of 256 lanes)
 
+// arithmetic classes: FP64 p-adic (p < 2^50), 32-bit Montgomery with 2 / 3 / 4 limbs,
+// 4 limbs keeping the carry bit (p >= 2^63)
+enum Arith { A_F64 = 0, A_M2, A_M3, A_M4, A_FULL };
+
 struct TestOut {
   DevAcc* acc;
   DevEvent* ev;
@@ -341,6 +390,15 @@ __device__ __forceinline__ void push_event(const TestOut& o, u32 type, u32 rid, 
     e.type = type; e.rid = rid; e.p = p; e.q = q; e.r_hi = rhi; e.r_lo = rlo;
     o.ev[k] = e;
   }
+}
+
+// 128-bit r = a + b p from the normalised p-adic digits
+__device__ __forceinline__ void padic_to_r(u64 p, double a, double b, u64& rhi, u64& rlo) {
+  const u64 ai = (u64)a, bi = (u64)b;
+  const u64 lo = bi * p;
+  const u64 hi = __umul64hi(bi, p);
+  rlo = lo + ai;
+  rhi = hi + (rlo < lo);
 }
 
 // Account one tested prime exactly as finish_lanes (wieferich.c:1051-1075) and add the
@@ -369,6 +427,32 @@ __device__ __forceinline__ void account(const TestOut& o, u32 rid, u64 p, u64 rh
   }
 }
 
+// The same from the p-adic digits of r = a + b p: for prime p, a = 1 and q = b exactly,
+// so p^-1 and r are only formed for an event or a failed FLT check.
+__device__ __forceinline__ void account_padic(const TestOut& o, u32 rid, u64 p, double a, double b, LaneSums& s) {
+  if (a == 1.0) {
+    const u64 q = (u64)b;                      // < p: the FLT check holds
+    s.tests++;
+    s.cks += d_mix64(q ^ (p * GOLD));
+    s.sumq += q;
+    if (q == 0) {
+      push_event(o, WF_EV_FOUND, rid, p, 0, 0, 1);
+    } else if (o.near_T > 0) {
+      const i64 A = q <= p / 2 ? (i64)q : -(i64)(p - q);
+      if ((A < 0 ? -A : A) <= o.near_T) {
+        s.near++;
+        const u64 lo = q * p, hi = __umul64hi(q, p);
+        const u64 rlo = lo + 1;
+        push_event(o, WF_EV_NEAR, rid, p, q, hi + (rlo < lo), rlo);
+      }
+    }
+    return;
+  }
+  u64 rhi, rlo;
+  padic_to_r(p, a, b, rhi, rlo);
+  account(o, rid, p, rhi, rlo, s);
+}
+
 // Wavefront sums of a range's lane totals -> one atomic per counter (lane 0).
 __device__ __forceinline__ void flush_sums(const TestOut& o, u32 rid, LaneSums& s) {
   const u64 t = wave_sum(s.tests), f = wave_sum(s.flt), n = wave_sum(s.near);
@@ -382,15 +466,6 @@ __device__ __forceinline__ void flush_sums(const TestOut& o, u32 rid, LaneSums& 
     atomicAdd(&a->sumq, q);
   }
   s.tests = s.flt = s.near = s.cks = s.sumq = 0;
-}
-
-// 128-bit r = a + b p from the normalised p-adic digits
-__device__ __forceinline__ void padic_to_r(u64 p, double a, double b, u64& rhi, u64& rlo) {
-  const u64 ai = (u64)a, bi = (u64)b;
-  const u64 lo = bi * p;
-  const u64 hi = __umul64hi(bi, p);
-  rlo = lo + ai;
-  rhi = hi + (rlo < lo);
 }
 
 // One prime per lane over the segments' prime lists; workgroups take segments in turn
@@ -434,14 +509,28 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
       cur = d.rid;
     }
     const u32* po = off + (u64)sg * pcap;
-    for (u32 base = beg; base < end; base += blockDim.x) {
-      const u32 i = base + threadIdx.x;
-      const bool active = i < end;
-      const u64 p = active ? d.base + po[i] : idle;
-      const int eb = wave_max(64 - __builtin_clzll(p - 1));
-      u64 rhi, rlo;
-      residue<ARITH>(p, eb, rhi, rlo);
-      if (active) account(o, d.rid, p, rhi, rlo, s);
+    if (ARITH == A_F64) {
+      // two primes per lane: i and i + blockDim.x
+      for (u32 base = beg; base < end; base += 2 * blockDim.x) {
+        const u32 i0 = base + threadIdx.x, i1 = i0 + blockDim.x;
+        const bool act0 = i0 < end, act1 = i1 < end;
+        const u64 p0 = act0 ? d.base + po[i0] : idle, p1 = act1 ? d.base + po[i1] : idle;
+        const int eb = wave_max(max(64 - __builtin_clzll(p0 - 1), 64 - __builtin_clzll(p1 - 1)));
+        double a0, b0, a1, b1;
+        fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, eb, a0, b0, a1, b1);
+        if (act0) account_padic(o, d.rid, p0, a0, b0, s);
+        if (act1) account_padic(o, d.rid, p1, a1, b1, s);
+      }
+    } else {
+      for (u32 base = beg; base < end; base += blockDim.x) {
+        const u32 i = base + threadIdx.x;
+        const bool active = i < end;
+        const u64 p = active ? d.base + po[i] : idle;
+        const int eb = wave_max(64 - __builtin_clzll(p - 1));
+        u64 rhi, rlo;
+        residue<ARITH>(p, eb, rhi, rlo);
+        if (active) account(o, d.rid, p, rhi, rlo, s);
+      }
     }
   }
   if (cur != 0xFFFFFFFFu) flush_sums(o, cur, s);
@@ -545,6 +634,8 @@ struct wf_engine {
   u64 seg_cap = 0;
   RunDesc* d_runs = nullptr;
   u64 run_cap = 0;
+  FillDesc* d_fills = nullptr;
+  u64 fill_cap = 0;
   BufSet bs[2];
   u32* d_flags = nullptr;
   u32* d_pre = nullptr;          // presieve pattern, 4 copies shifted by 0..3 bytes
@@ -588,7 +679,8 @@ static u64 isqrt_u64(u64 n) {
   return r;
 }
 
-// Odd primes 7 <= q <= lim on the host (segmented odd-only sieve), lim <= 2^32.
+// Odd primes 7 <= q <= lim on the host (segmented odd-only sieve, several threads),
+// lim <= 2^32.  Built once per engine and kept on the device.
 static void host_primes(u64 lim, std::vector<u32>& out) {
   out.clear();
   const u64 root = isqrt_u64(lim);
@@ -599,24 +691,40 @@ static void host_primes(u64 lim, std::vector<u32>& out) {
     if (i > 2) base.push_back((u32)i);
     for (u64 k = i * i; k <= root; k += i) small[k] = 0;
   }
-  const u64 SEGN = 1u << 22;  // odd numbers per segment
-  std::vector<uint8_t> comp(SEGN);
-  for (u64 lo = 1; lo <= lim; lo += 2 * SEGN) {  // odd numbers lo, lo + 2, ...
-    const u64 hi = std::min(lim + 1, lo + 2 * SEGN);
-    const u64 cnt = (hi - lo + 1) / 2;
-    std::fill(comp.begin(), comp.begin() + cnt, 0);
-    for (u32 q : base) {
-      const u64 qq = (u64)q * q;
-      if (qq >= hi) break;
-      u64 f = qq >= lo ? qq : lo + (q - lo % q) % q;
-      if (!(f & 1)) f += q;
-      for (u64 k = (f - lo) / 2; k < cnt; k += q) comp[k] = 1;
+  const u64 SEGN = 1u << 21;  // odd numbers per segment
+  const u64 nsegs = (lim + 2 * SEGN) / (2 * SEGN);
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (nsegs < 4 * nt) nt = 1;
+  std::vector<std::vector<u32>> part(nt);
+  auto work = [&](unsigned t) {
+    std::vector<uint8_t> comp(SEGN);
+    for (u64 sgi = t * nsegs / nt; sgi < (t + 1) * nsegs / nt; sgi++) {
+      const u64 lo = 1 + sgi * 2 * SEGN;         // odd numbers lo, lo + 2, ...
+      if (lo > lim) break;
+      const u64 hi = std::min(lim + 1, lo + 2 * SEGN);
+      const u64 cnt = (hi - lo + 1) / 2;
+      std::fill(comp.begin(), comp.begin() + cnt, 0);
+      for (u32 q : base) {
+        const u64 qq = (u64)q * q;
+        if (qq >= hi) break;
+        u64 f = qq >= lo ? qq : lo + (q - lo % q) % q;
+        if (!(f & 1)) f += q;
+        for (u64 k = (f - lo) / 2; k < cnt; k += q) comp[k] = 1;
+      }
+      for (u64 k = 0; k < cnt; k++) {
+        const u64 n = lo + 2 * k;
+        if (!comp[k] && n >= 7 && n <= lim) part[t].push_back((u32)n);
+      }
     }
-    for (u64 k = 0; k < cnt; k++) {
-      const u64 n = lo + 2 * k;
-      if (!comp[k] && n >= 7 && n <= lim) out.push_back((u32)n);
-    }
-  }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nt; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
+  size_t total = 0;
+  for (auto& v : part) total += v.size();
+  out.reserve(total);
+  for (auto& v : part) out.insert(out.end(), v.begin(), v.end());
 }
 
 static int ensure_sieving_primes(wf_engine* e, u64 lim) {
@@ -759,6 +867,7 @@ extern "C" void wf_close(wf_engine* e) {
   (void)hipFree(e->d_spinv);
   (void)hipFree(e->d_segs);
   (void)hipFree(e->d_runs);
+  (void)hipFree(e->d_fills);
   for (int k = 0; k < 2; k++) {
     (void)hipFree(e->bs[k].bcnt);
     (void)hipFree(e->bs[k].bdata);
@@ -800,7 +909,6 @@ extern "C" int wf_set_bases(wf_engine* e, const u32* bases, int nb) {
 }
 
 // A sub-range of one arithmetic class: [lo, hi) (hi == 0 means 2^64) with its kernel.
-enum Arith { A_F64 = 0, A_M2, A_M3, A_M4, A_FULL };
 struct Piece { u64 lo, hi; u32 rid; int arith; };
 
 // Split [lo, hi) at the class boundaries of the arithmetic paths.
@@ -877,6 +985,7 @@ static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vec
 struct PassPlan {
   std::vector<SegDesc> segs;
   std::vector<RunDesc> runs;
+  std::vector<FillDesc> fills;
   u32 maxnq = 0, bcap = 0, pcap = 0, tps = 0;
 };
 
@@ -889,11 +998,50 @@ static void plan_pass(const wf_engine* e, const std::vector<Piece>& pieces, Pass
   P.pcap = (u32)prime_bound(30.0 * e->cfg.seg_bytes);
   P.tps = (P.pcap + TEST_TILE - 1) / TEST_TILE;
   P.bcap = 0;
+  P.fills.clear();
   if (P.maxnq > e->q1idx) {
     // expected records per segment: sum over large primes and classes of seg_bytes / q; the
     // count is a sum of independent 0/1 (or floor/ceil) terms, so its deviation is ~sqrt(E)
     const double E = 8.0 * e->cfg.seg_bytes * (e->h_inv_sum[P.maxnq] - e->h_inv_sum[e->q1idx]);
     P.bcap = (u32)((E + 10.0 * std::sqrt(E) + 1024.0 + 63.0) / 64.0) * 64;
+    // fill work items.  Primes in [Q, 2Q) walk runs of G segments, G the power of two
+    // (<= RUN_MAX) nearest 64 Q / (8 seg_bytes): about 64 records per prime and run, so
+    // the per-run start of a prime is paid for by its hits.  Each run's primes are cut
+    // into slices of ~FILL_TARGET expected records.
+    const double SBd = e->cfg.seg_bytes;
+    for (const RunDesc& ch : P.runs) {
+      if (ch.nq <= e->q1idx) continue;
+      u32 i = e->q1idx;
+      while (i < ch.nq) {
+        const u32 Q = e->h_sp[i];
+        const u32 qend = Q >= 0x80000000u ? 0xFFFFFFFFu : 2 * Q;
+        const u32 bend = std::min<u32>(ch.nq, (u32)(std::lower_bound(e->h_sp.begin() + i, e->h_sp.end(), qend) -
+                                                     e->h_sp.begin()));
+        u32 G = 1;
+        while (G < RUN_MAX && 2.0 * G * 8.0 * SBd <= 64.0 * Q) G *= 2;
+        for (u32 s0 = 0; s0 < ch.nseg; s0 += G) {
+          const u32 ns = std::min(G, ch.nseg - s0);
+          const double per = 8.0 * SBd * ns;     // expected records of prime q: per / q
+          u32 a = i;
+          while (a < bend) {
+            const double lim = e->h_inv_sum[a] + FILL_TARGET / per;
+            u32 b = (u32)(std::upper_bound(e->h_inv_sum.begin() + a + 1, e->h_inv_sum.begin() + bend + 1, lim) -
+                          e->h_inv_sum.begin()) - 1;
+            if (b <= a) b = a + 1;
+            b = std::min(b, std::min<u32>(bend, a + 65536));
+            FillDesc f;
+            f.base = ch.base + (u64)s0 * 30ull * e->cfg.seg_bytes;
+            f.seg0 = ch.seg0 + s0;
+            f.nseg = ns;
+            f.i0 = a;
+            f.i1 = b;
+            P.fills.push_back(f);
+            a = b;
+          }
+        }
+        i = bend;
+      }
+    }
   }
 }
 
@@ -911,16 +1059,17 @@ static int size_set(BufSet& B, const PassPlan& P) {
 }
 
 // Bucket fill + sieve of pass P (descriptors at dsegs / druns) into buffer set B on stream st.
-static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, const RunDesc* druns, BufSet& B,
-                         hipStream_t st, DevAcc* acc, bool want_bitmap = false) {
+static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, const RunDesc* druns,
+                         const FillDesc* dfills, BufSet& B, hipStream_t st, DevAcc* acc, bool want_bitmap = false) {
   const u64 nseg = P.segs.size();
   const int cus = e->prop.multiProcessorCount;
   if (P.bcap) {
     hipLaunchKernelGGL(k_zero, dim3((u32)std::min<u64>((nseg + 255) / 256, 1024)), dim3(256), 0, st, B.bcnt, nseg);
-    const u32 npb = (P.maxnq - e->q1idx + BF_PRIMES - 1) / BF_PRIMES;
-    hipLaunchKernelGGL(k_bucket_fill, dim3((u32)(P.runs.size() * npb)), dim3(256), 0, st, druns, npb, e->d_sp,
-                       e->d_spinv, e->q1idx, B.bcnt, B.bdata, P.bcap, e->sbshift, e->d_flags);
-    HIPCHK(hipGetLastError());
+    if (!P.fills.empty()) {
+      hipLaunchKernelGGL(k_bucket_fill, dim3((u32)P.fills.size()), dim3(256), 0, st, dfills, e->d_sp,
+                         e->d_spinv, B.bcnt, B.bdata, P.bcap, e->sbshift, e->d_flags);
+      HIPCHK(hipGetLastError());
+    }
   }
   const u32 sgrid = (u32)std::min<u64>(nseg, (u64)cus * 4);
   hipLaunchKernelGGL(k_sieve, dim3(sgrid), dim3(e->cfg.sieve_block), e->cfg.seg_bytes, st, dsegs, (u32)nseg, e->d_sp,
@@ -956,6 +1105,10 @@ static int upload_pass(wf_engine* e, const PassPlan& P) {
   if ((rc = grow(e->d_runs, e->run_cap, std::max<size_t>(P.runs.size(), 1))) != WF_OK) return rc;
   HIPCHK(hipMemcpyAsync(e->d_segs, P.segs.data(), P.segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, e->st));
   HIPCHK(hipMemcpyAsync(e->d_runs, P.runs.data(), P.runs.size() * sizeof(RunDesc), hipMemcpyHostToDevice, e->st));
+  if (!P.fills.empty()) {
+    if ((rc = grow(e->d_fills, e->fill_cap, P.fills.size())) != WF_OK) return rc;
+    HIPCHK(hipMemcpyAsync(e->d_fills, P.fills.data(), P.fills.size() * sizeof(FillDesc), hipMemcpyHostToDevice, e->st));
+  }
   return WF_OK;
 }
 
@@ -971,7 +1124,7 @@ static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith,
     return rc;
   if ((rc = upload_pass(e, P)) != WF_OK) return rc;
   HIPCHK(hipEventRecord(e->ev[0], e->st));
-  if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->bs[0], e->st, to.acc, want_psp)) != WF_OK) return rc;
+  if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->d_fills, e->bs[0], e->st, to.acc, want_psp)) != WF_OK) return rc;
   HIPCHK(hipEventRecord(e->ev[1], e->st));
   if ((rc = enqueue_test(e, P, e->d_segs, e->bs[0], e->st, arith, to)) != WF_OK) return rc;
   HIPCHK(hipEventRecord(e->ev[2], e->st));
@@ -1002,20 +1155,26 @@ static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith,
 static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Piece>, int>>& passes,
                        const TestOut& to) {
   std::vector<PassPlan> plans(passes.size());
-  std::vector<u64> soff(passes.size()), roff(passes.size());
-  u64 nsegs = 0, nruns = 0;
+  std::vector<u64> soff(passes.size()), roff(passes.size()), foff(passes.size());
+  u64 nsegs = 0, nruns = 0, nfills = 0;
   for (size_t k = 0; k < passes.size(); k++) {
     plan_pass(e, passes[k].first, plans[k]);
     soff[k] = nsegs;
     roff[k] = nruns;
+    foff[k] = nfills;
     nsegs += plans[k].segs.size();
     nruns += plans[k].runs.size();
+    nfills += plans[k].fills.size();
   }
   if (nsegs == 0) return WF_OK;
   int rc;
   if ((rc = grow(e->d_segs, e->seg_cap, nsegs)) != WF_OK) return rc;
   if ((rc = grow(e->d_runs, e->run_cap, std::max<u64>(nruns, 1))) != WF_OK) return rc;
+  if ((rc = grow(e->d_fills, e->fill_cap, std::max<u64>(nfills, 1))) != WF_OK) return rc;
   for (size_t k = 0; k < plans.size(); k++) {
+    if (!plans[k].fills.empty())
+      HIPCHK(hipMemcpyAsync(e->d_fills + foff[k], plans[k].fills.data(), plans[k].fills.size() * sizeof(FillDesc),
+                            hipMemcpyHostToDevice, e->st));
     if ((rc = size_set(e->bs[k & 1], plans[k])) != WF_OK) return rc;
     HIPCHK(hipMemcpyAsync(e->d_segs + soff[k], plans[k].segs.data(), plans[k].segs.size() * sizeof(SegDesc),
                           hipMemcpyHostToDevice, e->st));
@@ -1035,7 +1194,9 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
     const int set = (int)(k & 1);
     hipEvent_t sieved = e->gev[1 + set], tested = e->gev[3 + set];
     if (k >= 2 && hipStreamWaitEvent(e->st, tested, 0) != hipSuccess) { rc = WF_E_DEVICE; break; }
-    if ((rc = enqueue_sieve(e, P, e->d_segs + soff[k], e->d_runs + roff[k], e->bs[set], e->st, to.acc)) != WF_OK) break;
+    if ((rc = enqueue_sieve(e, P, e->d_segs + soff[k], e->d_runs + roff[k], e->d_fills + foff[k], e->bs[set], e->st,
+                            to.acc)) != WF_OK)
+      break;
     if (hipEventRecord(sieved, e->st) != hipSuccess || hipStreamWaitEvent(e->st2, sieved, 0) != hipSuccess) {
       rc = WF_E_DEVICE;
       break;
@@ -1254,7 +1415,7 @@ extern "C" int wf_primes(wf_engine* e, u64 lo, u64 hi, u64* out, u64 cap, u64* c
     plan_pass(e, pieces, P);
     if ((rc = size_set(e->bs[0], P)) != WF_OK) return rc;
     if ((rc = upload_pass(e, P)) != WF_OK) return rc;
-    if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->bs[0], e->st, e->d_acc)) != WF_OK) return rc;
+    if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->d_fills, e->bs[0], e->st, e->d_acc)) != WF_OK) return rc;
     std::vector<u32> cnt(P.segs.size());
     u32 flags = 0;
     HIPCHK(hipMemcpyAsync(cnt.data(), e->bs[0].cnt, cnt.size() * sizeof(u32), hipMemcpyDeviceToHost, e->st));
@@ -1302,7 +1463,8 @@ extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double
   const wf_config orig = e->cfg;
   wf_config bestc = orig;
   double bestt = 1e300;
-  const u32 segs[] = {16384, 32768, 65536}, blocks[] = {256, 512, 1024}, bmins[] = {32768, 65536, 131072, 262144};
+  const u32 segs[] = {16384, 32768, 65536}, blocks[] = {256, 512, 1024},
+            bmins[] = {32768, 65536, 131072, 262144, 524288, 1048576};
   for (u32 sb : segs)
     for (u32 bl : blocks)
       for (u32 bm : bmins) {
@@ -1321,7 +1483,7 @@ extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double
         float ms = 0;
         for (int rep = 0; rep < 2; rep++) {   // the first run warms caches and code
           HIPCHK(hipEventRecord(e->ev[0], e->st));
-          if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->bs[0], e->st, e->d_acc)) != WF_OK) {
+          if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->d_fills, e->bs[0], e->st, e->d_acc)) != WF_OK) {
             apply_config(e, orig);
             return rc;
           }

@@ -167,3 +167,19 @@ def test_serving_same_alone_and_in_mix():
         g = got[i]
         assert (g["primes"], g["tests"], g["cks"], g["sumq"], g["flt"], g["nev"]) == \
             (w["primes"], w["tests"], w["cks"], w["sumq"], w["flt"], w["nev"]), ranges[i]
+
+
+def test_tiny_and_empty_ranges(eng):
+    # one-number ranges, ranges without primes, ranges ending at 2^64, and a rejected empty range
+    cases = [(2, 3), (3, 4), (4, 5), (24, 29), (90, 97), (1093, 1094), (10 ** 15 + 37, 10 ** 15 + 38),
+             ((1 << 64) - 59, 1 << 64), ((1 << 64) - 1, 1 << 64), ((1 << 63) - 1, 1 << 63)]
+    for lo, hi in cases:
+        r = eng.scan(lo, hi)
+        o = oracle.scan(lo, hi, bases=[2])
+        assert (r["primes"], r["tests"], r["cks"]) == (o["primes"], o["tests"], o["cks"][2]), (lo, hi)
+    with pytest.raises(wf.WFError):
+        eng.scan(100, 100)
+    per, ev = eng.scan_batch(cases)
+    for (lo, hi), r in zip(cases, per):
+        o = oracle.scan(lo, hi, bases=[2])
+        assert (r["primes"], r["cks"]) == (o["primes"], o["cks"][2]), (lo, hi)
