@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--queries", type=int, default=2000, help="c5: queries per rate")
     ap.add_argument("--rates", default="100,1000,5000", help="c5: Poisson arrival rates per second")
     ap.add_argument("--c4-span", type=float, default=2.0 ** 34, help="c4: numbers below 2^64 per step")
+    ap.add_argument("--test-wgs", type=int, default=0,
+                    help="batched: test workgroups per CU (0 = engine default)")
     args = ap.parse_args()
     if args.workload != "c2" and args.gpus == 1:
         return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
@@ -238,7 +240,7 @@ def main():
         return 2
     from mathpub_amd import wf
     span = int(args.span)
-    eng = wf.Engine(arith=args.arith)
+    eng = wf.Engine(arith=args.arith, test_wgs=args.test_wgs)
     print("engine: " + eng.info(), file=sys.stderr, flush=True)
 
     def timed(fn, label):

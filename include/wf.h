@@ -79,6 +79,8 @@ typedef struct {
     uint32_t arith;         /* 0 = fastest exact path, 1 = force 32-bit Montgomery */
     uint32_t bucket_min;    /* sieving primes >= this go through bucket records (>= 512) */
     uint64_t chunk;         /* numbers per device pass of wf_scan (memory bound) */
+    uint32_t test_wgs;      /* wf_scan_batch: test workgroups (4 wavefronts) per CU, 1 .. 16;
+                               fewer leave room for the next pass's sieve beside the tests */
 } wf_config;
 
 /* Engine life cycle (the reference's gpu_init, wieferich_cuda.cu:1570-1577). */

@@ -337,6 +337,125 @@ __device__ __forceinline__ void mont_pow2(u64 p, u64 e, int ebits, u32 (&r)[L]) 
   for (int i = 0; i < L; i++) r[i] = x[i];
 }
 
+// ---------------------------------------------------------------------------------
+// radix-2^28 Montgomery, 5 limbs, R = 2^140 (any odd p with 2^47 <= p < 2^64)
+// ---------------------------------------------------------------------------------
+// Limbs of 28 bits in 32-bit registers; every partial product (< 2^59) is one
+// v_mad_u64_u32 into a 64-bit column sum, and carries move once per column, so no
+// product needs its own carry chain.  N = p^2 < 2^128 leaves 12 spare bits below R:
+// any input x < 64 N squares to (x^2 + m N) / R < N (4096 N / R + 1) < 2N, so the base-2
+// ladder doubles without reducing (limbs may then hold 29 bits: products stay < 2^59 and
+// column sums < 2^62) and only the final REDC reduces fully.  This also covers p >= 2^63,
+// where N reaches 2^128 and a 32-bit-limb REDC needs its 129th bit.
+
+#define M28 0x0FFFFFFFu
+
+// One REDC row: clear column T0 (mod 2^28) with m N, carry T0 into T1.
+#define M28_ROW(T0, T1, T2, T3, T4)                  \
+  m = ((u32)T0 * ninv) & M28;                      \
+  T0 += (u64)m * n[0];                             \
+  T1 += (u64)m * n[1] + (T0 >> 28);                \
+  T2 += (u64)m * n[2];                             \
+  T3 += (u64)m * n[3];                             \
+  T4 += (u64)m * n[4];
+
+// x = x^2 / R mod N (lazy: input < 64 N, result < 2N with normalised limbs)
+__device__ __forceinline__ void m28_sqr(u32 (&x)[5], const u32 (&n)[5], u32 ninv) {
+  const u32 d1 = x[1] << 1, d2 = x[2] << 1, d3 = x[3] << 1, d4 = x[4] << 1;
+  u64 t0 = (u64)x[0] * x[0];
+  u64 t1 = (u64)x[0] * d1;
+  u64 t2 = (u64)x[0] * d2 + (u64)x[1] * x[1];
+  u64 t3 = (u64)x[0] * d3 + (u64)x[1] * d2;
+  u64 t4 = (u64)x[0] * d4 + (u64)x[1] * d3 + (u64)x[2] * x[2];
+  u64 t5 = (u64)x[1] * d4 + (u64)x[2] * d3;
+  u64 t6 = (u64)x[2] * d4 + (u64)x[3] * x[3];
+  u64 t7 = (u64)x[3] * d4;
+  u64 t8 = (u64)x[4] * x[4];
+  u32 m;
+  M28_ROW(t0, t1, t2, t3, t4)
+  M28_ROW(t1, t2, t3, t4, t5)
+  M28_ROW(t2, t3, t4, t5, t6)
+  M28_ROW(t3, t4, t5, t6, t7)
+  M28_ROW(t4, t5, t6, t7, t8)
+  x[0] = (u32)t5 & M28;
+  t6 += t5 >> 28;
+  x[1] = (u32)t6 & M28;
+  t7 += t6 >> 28;
+  x[2] = (u32)t7 & M28;
+  t8 += t7 >> 28;
+  x[3] = (u32)t8 & M28;
+  x[4] = (u32)(t8 >> 28);
+}
+
+// REDC of x < 64 N: x / R mod N, fully reduced, as four 32-bit words.
+__device__ __forceinline__ void m28_out(const u32 (&x)[5], const u32 (&n)[5], u32 ninv, u32 (&r)[4]) {
+  u64 t0 = x[0], t1 = x[1], t2 = x[2], t3 = x[3], t4 = x[4], t5 = 0, t6 = 0, t7 = 0, t8 = 0;
+  u32 m;
+  M28_ROW(t0, t1, t2, t3, t4)
+  M28_ROW(t1, t2, t3, t4, t5)
+  M28_ROW(t2, t3, t4, t5, t6)
+  M28_ROW(t3, t4, t5, t6, t7)
+  M28_ROW(t4, t5, t6, t7, t8)
+  t6 += t5 >> 28;
+  t7 += t6 >> 28;
+  t8 += t7 >> 28;
+  // y = (x + m N) / R <= N, in limbs t5..t8 (t8 < 2^44)
+  const unsigned __int128 y = ((unsigned __int128)t8 << 84) | ((unsigned __int128)((u32)t7 & M28) << 56) |
+                              ((unsigned __int128)((u32)t6 & M28) << 28) | ((u32)t5 & M28);
+  const unsigned __int128 N = ((unsigned __int128)n[4] << 112) | ((unsigned __int128)n[3] << 84) |
+                              ((unsigned __int128)n[2] << 56) | ((unsigned __int128)n[1] << 28) | n[0];
+  const unsigned __int128 z = y >= N ? y - N : y;
+  r[0] = (u32)z;
+  r[1] = (u32)(z >> 32);
+  r[2] = (u32)(z >> 64);
+  r[3] = (u32)(z >> 96);
+}
+#undef M28_ROW
+
+// r = 2^e mod N = p^2 as four 32-bit words, for odd p >= 2^47.  ebits = the wave's longest
+// exponent length (shorter exponents skip their leading positions).
+__device__ __forceinline__ void m28_pow2(u64 p, u64 e, int ebits, u32 (&r)[4]) {
+  typedef unsigned __int128 u128;
+  const u128 N = (u128)p * p;
+  u32 n[5];
+  for (int i = 0; i < 5; i++) n[i] = (u32)(N >> (28 * i)) & M28;
+  const u64 pinv = inv64(p);
+  const u32 ninv = (0u - (u32)pinv * (u32)pinv) & M28;
+  // R mod N = 2^140 - Q N with Q = floor(2^140 / N) < 2^46 estimated in FP64 (relative
+  // error < 2^-50, so off by at most one); the 192-bit difference is then brought into
+  // [0, N) by at most two additions or subtractions of N.
+  const u64 Q = (u64)(1393796574908163946345982392040522594123776.0 / ((double)p * (double)p));
+  const u128 A = (u128)Q * (u64)N;                   // Q * N_lo
+  const u128 B = (u128)Q * (u64)(N >> 64);           // Q * N_hi
+  const u128 low = A + (B << 64);                    // Q N mod 2^128
+  const long long top = (long long)((u64)(B >> 64) + (low < A ? 1u : 0u));   // Q N >> 128
+  u128 vlow = (u128)0 - low;                         // 2^140 - Q N = vtop 2^128 + vlow
+  long long vtop = 4096 - top - (low != 0 ? 1 : 0);
+  while (vtop < 0) {
+    const u128 t = vlow + N;
+    vtop += t < vlow ? 1 : 0;
+    vlow = t;
+  }
+  while (vtop > 0 || vlow >= N) {
+    vtop -= vlow < N ? 1 : 0;
+    vlow -= N;
+  }
+  // Montgomery form of 2: 2 (R mod N) < 2N
+  const u128 two = vlow << 1;                         // < 2^129: bit 128 is lost only if N >= 2^127
+  u32 x[5];
+  for (int i = 0; i < 4; i++) x[i] = (u32)(two >> (28 * i)) & M28;
+  x[4] = (u32)(two >> 112) | ((vlow >> 127) ? (1u << 16) : 0u);
+  // ladder over the bits below this lane's top bit
+  const int tb = 63 - __builtin_clzll(e);
+  for (int i = ebits - 2; i >= 0; i--) {
+    if (i >= tb) continue;
+    m28_sqr(x, n, ninv);
+    const u32 bit = (u32)(e >> i) & 1u;
+    x[0] <<= bit; x[1] <<= bit; x[2] <<= bit; x[3] <<= bit; x[4] <<= bit;
+  }
+  m28_out(x, n, ninv, r);
+}
+
 }  // namespace wf
 
 namespace wf {

@@ -83,14 +83,26 @@ __constant__ unsigned char c_M30[8 * 30 * 8];
 // c_QI[q mod 30] = index of q mod 30 in R30 (q coprime to 30)
 __constant__ unsigned char c_QI[30];
 
+// The two wheel tables, copied to LDS by every workgroup that looks them up per lane
+// (per-lane lookups in constant memory are vector memory loads with cache latency).
+struct WheelLds {
+  unsigned char m30[8 * 30 * 8];
+  unsigned char qi[32];
+};
+
+__device__ __forceinline__ void load_wheel(WheelLds& w) {
+  for (u32 k = threadIdx.x; k < 8 * 30 * 8; k += blockDim.x) w.m30[k] = c_M30[k];
+  if (threadIdx.x < 30) w.qi[threadIdx.x] = c_QI[threadIdx.x];
+}
+
 // Byte index j (relative to the segment base) of the first mark of class c for a prime q
 // whose first multiple at or after max(base, q^2) lies at offset o (numbers) from base.
 // o < 2^32 at every call (o < q, or o < the span of a segment or fill run).  The
 // numerator is a multiple of 30, so below 30 * 2^32 the division is exact: halve, then
 // multiply by 15^-1 mod 2^32.  Larger starts lie beyond every segment and run.
-__device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c) {
+__device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c, const WheelLds& w) {
   const u32 o32 = (u32)o;
-  const u32 m = c_M30[(qi * 30 + o32 % 30u) * 8 + c];
+  const u32 m = w.m30[(qi * 30 + o32 % 30u) * 8 + c];
   const u64 v = (u64)m * q + o32 - c_R30[c];
   if (v >= (30ull << 32)) return ~0ull;   // beyond any segment or run (q close to 2^32)
   return (u64)((u32)(v >> 1) * 0xEEEEEEEFu);
@@ -108,10 +120,17 @@ __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c) {
 // (byte << 3 | class) per hit in the bucket of the segment it lands in.
 
 #define RUN_MAX 1024      // segments per run (fill runs are cut shorter for smaller primes)
+#define SEG_MAX 131072    // largest LDS segment in bytes (a bucket record is byte << 3 | class)
 #define PRE_PERIOD 323323u  // 7 * 11 * 13 * 17 * 19: period of the presieve pattern in wheel bytes
 #define PRE_COUNT 5u        // the presieve primes are the first five sieving primes
 #define FILL_STAGE 8192   // records a fill workgroup stages in LDS (32 KiB)
 #define FILL_TARGET 4096  // expected records per fill work item (the host sizes items to it)
+#ifndef FILL_BLOCK
+#define FILL_BLOCK 512    // threads per fill workgroup
+#endif
+#ifndef SIEVE_WAVES_EU
+#define SIEVE_WAVES_EU 8  // sieve register budget: 8 wavefronts per SIMD (<= 64 VGPRs, no spills)
+#endif
 
 struct SegDesc {
   u64 base;       // multiple of 30
@@ -155,12 +174,15 @@ __device__ __forceinline__ bool first_offset(u64 base, u32 q, u64 minv, u64 span
 // segment reserves room in its bucket; walk 2 stages the records in LDS grouped by
 // segment; the workgroup then copies each group to its bucket with consecutive lanes on
 // consecutive addresses.  Records that do not fit the stage go straight to the bucket.
-__global__ void __launch_bounds__(256) k_bucket_fill(const FillDesc* __restrict__ items,
+__global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __restrict__ items,
                                                      const u32* __restrict__ sp, const u64* __restrict__ spinv,
                                                      u32* __restrict__ bcnt, u32* __restrict__ bdata,
                                                      u32 bcap, int sbshift, u32* __restrict__ flags) {
   __shared__ u32 hist[RUN_MAX], loff[RUN_MAX + 1], gbase[RUN_MAX];
   __shared__ u32 stage[FILL_STAGE];
+  __shared__ u32 wtot[FILL_BLOCK / 64];
+  __shared__ WheelLds wl;
+  load_wheel(wl);
   const FillDesc it = items[blockIdx.x];
   const FillDesc& r = it;
   const u64 nbytes = (u64)r.nseg << sbshift;
@@ -176,19 +198,29 @@ __global__ void __launch_bounds__(256) k_bucket_fill(const FillDesc* __restrict_
     const u32 q = sp[i];
     u64 o;
     if (!first_offset(r.base, q, spinv[i], span, o)) continue;
-    const int qi = c_QI[q % 30];
+    const int qi = wl.qi[q % 30];
     const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
     for (int c = c0; c < c1; c++)
-      for (u64 j = class_start(o, q, qi, c); j < nbytes; j += q) atomicAdd(&hist[j >> sbshift], 1u);
+      for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q) atomicAdd(&hist[j >> sbshift], 1u);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    u32 acc = 0;
-    for (u32 k = 0; k < r.nseg; k++) {
+  // loff = exclusive prefix sums of hist: each thread scans a few consecutive segments,
+  // the wavefront and workgroup totals give its starting point
+  {
+    const u32 per = (r.nseg + blockDim.x - 1) / blockDim.x;
+    const u32 k0 = min(threadIdx.x * per, r.nseg), k1 = min(k0 + per, r.nseg);
+    u32 local = 0;
+    for (u32 k = k0; k < k1; k++) local += hist[k];
+    const u32 incl = wave_incl_scan(local);
+    if ((threadIdx.x & 63) == 63) wtot[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    u32 acc = incl - local;
+    for (u32 w = 0; w < (threadIdx.x >> 6); w++) acc += wtot[w];
+    for (u32 k = k0; k < k1; k++) {
       loff[k] = acc;
       acc += hist[k];
     }
-    loff[r.nseg] = acc;
+    if (threadIdx.x == blockDim.x - 1) loff[r.nseg] = acc;
   }
   __syncthreads();
   for (u32 k = threadIdx.x; k < r.nseg; k += blockDim.x) {
@@ -207,15 +239,15 @@ __global__ void __launch_bounds__(256) k_bucket_fill(const FillDesc* __restrict_
     const u32 q = sp[i];
     u64 o;
     if (!first_offset(r.base, q, spinv[i], span, o)) continue;
-    const int qi = c_QI[q % 30];
+    const int qi = wl.qi[q % 30];
     const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
     for (int c = c0; c < c1; c++)
-      for (u64 j = class_start(o, q, qi, c); j < nbytes; j += q) {
+      for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q) {
         const u32 sg = (u32)(j >> sbshift);
         const u32 rec = (((u32)j & smask) << 3) | (u32)c;
         const u32 slot = atomicAdd(&hist[sg], 1u);
         if (slot < FILL_STAGE) {
-          stage[slot] = (sg << 19) | rec;
+          stage[slot] = (sg << 20) | rec;
         } else {
           const u32 pos = gbase[sg] + (slot - loff[sg]);
           if (pos < bcap) bdata[(u64)(r.seg0 + sg) * bcap + pos] = rec;
@@ -226,9 +258,9 @@ __global__ void __launch_bounds__(256) k_bucket_fill(const FillDesc* __restrict_
   const u32 nst = min(loff[r.nseg], (u32)FILL_STAGE);
   for (u32 k = threadIdx.x; k < nst; k += blockDim.x) {
     const u32 v = stage[k];
-    const u32 sg = v >> 19;
+    const u32 sg = v >> 20;
     const u32 pos = gbase[sg] + (k - loff[sg]);
-    if (pos < bcap) bdata[(u64)(r.seg0 + sg) * bcap + pos] = v & 0x7FFFFu;
+    if (pos < bcap) bdata[(u64)(r.seg0 + sg) * bcap + pos] = v & 0xFFFFFu;
   }
 }
 
@@ -236,7 +268,8 @@ __global__ void __launch_bounds__(256) k_bucket_fill(const FillDesc* __restrict_
 __device__ __forceinline__ u32 word_mask(u32 w, u32 lo_off, u32 hi_off) {
   const u32 w0 = 120u * w;
   if (w0 >= lo_off && w0 + 120u <= hi_off) return 0xFFFFFFFFu;
-  u32 m = 0;
+  u32 m = 0;   // boundary words only: a rolled loop keeps the register count down
+#pragma unroll 1
   for (int b = 0; b < 32; b++) {
     const u32 off = w0 + 30u * (b >> 3) + c_R30[b & 7];
     if (off >= lo_off && off < hi_off) m |= 1u << b;
@@ -246,6 +279,7 @@ __device__ __forceinline__ u32 word_mask(u32 w, u32 lo_off, u32 hi_off) {
 
 // Sieve one segment per workgroup iteration and write its primes, in ascending order,
 // as offsets from the segment base into out[s * pcap ...]; cnt[s] = how many.
+__attribute__((amdgpu_waves_per_eu(SIEVE_WAVES_EU)))
 __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs, u32 nseg,
                                                 const u32* __restrict__ sp, const u64* __restrict__ spinv,
                                                 u32 tierA, u32 q1idx,
@@ -257,6 +291,8 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
                                                 const u32* __restrict__ pre, u32 pre_words) {
   extern __shared__ u32 seg[];
   __shared__ u32 wtot[16];
+  __shared__ WheelLds wl;
+  load_wheel(wl);   // the first barrier below orders it before every lookup
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
@@ -279,30 +315,42 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
       const u32 q = sp[i];
       u64 o;
       if (!first_offset(d.base, q, spinv[i], span, o)) continue;
-      const int qi = c_QI[q % 30];
-      const u32 jl = (u32)class_start(o, q, qi, lane & 7);     // lane c (< 8) holds class c's start
+      const int qi = wl.qi[q % 30];
+      const u32 jl = (u32)class_start(o, q, qi, lane & 7, wl);     // lane c (< 8) holds class c's start
       for (int c = 0; c < 8; c++) {
         const u32 j0 = __shfl(jl, c, 64);
         for (u32 j = j0 + (u32)lane * q; j < d.nbytes; j += 64u * q)
           atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c));
       }
     }
-    // tier B: one lane per prime
-    for (u32 i = tierA + threadIdx.x; i < nsmall; i += blockDim.x) {
-      const u32 q = sp[i];
-      u64 o;
-      if (!first_offset(d.base, q, spinv[i], span, o)) continue;
-      const int qi = c_QI[q % 30];
-#pragma unroll
-      for (int c = 0; c < 8; c++) {
-        for (u32 j = (u32)class_start(o, q, qi, c); j < d.nbytes; j += q)
-          atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c));
+    // tier B: one lane per prime; the next prime's q and Barrett constant are loaded
+    // while this one is sieved
+    {
+      u32 i = tierA + threadIdx.x;
+      u32 qn = i < nsmall ? sp[i] : 0;
+      u64 mn = i < nsmall ? spinv[i] : 0;
+      for (; i < nsmall; i += blockDim.x) {
+        const u32 q = qn;
+        const u64 minv = mn;
+        if (i + blockDim.x < nsmall) {
+          qn = sp[i + blockDim.x];
+          mn = spinv[i + blockDim.x];
+        }
+        u64 o;
+        if (!first_offset(d.base, q, minv, span, o)) continue;
+        const int qi = wl.qi[q % 30];
+#pragma unroll 1
+        for (int c = 0; c < 8; c++) {
+          for (u32 j = (u32)class_start(o, q, qi, c, wl); j < d.nbytes; j += q)
+            atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c));
+        }
       }
     }
     // large primes: this segment's bucket
     if (d.nq > q1idx) {
       const u32 nrec = min(bcnt[s], bcap);
       const u32* rec = bdata + (u64)s * bcap;
+#pragma unroll 4
       for (u32 k = threadIdx.x; k < nrec; k += blockDim.x) {
         const u32 v = rec[k];
         atomicOr(&seg[v >> 5], 1u << ((((v >> 3) & 3) << 3) | (v & 7)));
@@ -371,8 +419,8 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
 #define TEST_TILE 1024   // primes per test work item (4 rounds of 256 lanes)
 
 // arithmetic classes: FP64 p-adic (p < 2^50), 32-bit Montgomery with 2 / 3 / 4 limbs,
-// 4 limbs keeping the carry bit (p >= 2^63)
-enum Arith { A_F64 = 0, A_M2, A_M3, A_M4, A_FULL };
+// 4 limbs keeping the carry bit (p >= 2^63), radix-2^28 Montgomery (p >= 2^47)
+enum Arith { A_F64 = 0, A_M2, A_M3, A_M4, A_FULL, A_M28, A_NCLS };
 
 struct TestOut {
   DevAcc* acc;
@@ -477,6 +525,11 @@ __device__ __forceinline__ void residue(u64 p, int eb, u64& rhi, u64& rlo) {
     double a, b;
     fp64_pow2<5>(p, e, eb, a, b);
     padic_to_r(p, a, b, rhi, rlo);
+  } else if (ARITH == A_M28) {
+    u32 x[4];
+    m28_pow2(p, e, eb, x);
+    rlo = (u64)x[0] | ((u64)x[1] << 32);
+    rhi = (u64)x[2] | ((u64)x[3] << 32);
   } else {
     constexpr int L = ARITH == 1 ? 2 : (ARITH == 2 ? 3 : 4);
     constexpr bool FULL = ARITH == 4;
@@ -495,7 +548,7 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
   u32 cur = 0xFFFFFFFFu;
   // a valid modulus of the class for idle lanes
   const u64 idle = ARITH == 0 ? 3ull : ARITH == 1 ? 3ull : ARITH == 2 ? 2147483659ull
-                 : ARITH == 3 ? 140737488355333ull : 0xFFFFFFFFFFFFFFC5ull;
+                 : ARITH == 3 ? 140737488355333ull : 0xFFFFFFFFFFFFFFC5ull;   // M4 / FULL / M28
   const u64 ntiles = (u64)nseg * tps;
   for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const u32 sg = (u32)(t / tps);
@@ -536,17 +589,19 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
   if (cur != 0xFFFFFFFFu) flush_sums(o, cur, s);
 }
 
-// Quotients of arbitrary odd p (check / selftest): per-lane choice of the path.
+// Quotients of arbitrary odd p (check / selftest): per-lane choice of the path, the
+// scan's classes (int32 = 1 selects the 32-bit Montgomery ladders above 2^50).
 __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u64* __restrict__ rhi_out,
-                       u64* __restrict__ rlo_out, u64 n) {
+                       u64* __restrict__ rlo_out, u64 n, int int32) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const u64 p = p_in[i];
   const int eb = 64 - __builtin_clzll(p - 1);
   u64 rhi, rlo;
-  if (p < (1ull << 50)) residue<0>(p, eb, rhi, rlo);
-  else if (p < (1ull << 63)) residue<3>(p, eb, rhi, rlo);
-  else residue<4>(p, eb, rhi, rlo);
+  if (p < (1ull << 50)) residue<A_F64>(p, eb, rhi, rlo);
+  else if (!int32) residue<A_M28>(p, eb, rhi, rlo);
+  else if (p < (1ull << 63)) residue<A_M4>(p, eb, rhi, rlo);
+  else residue<A_FULL>(p, eb, rhi, rlo);
   q_out[i] = (rlo - 1) * inv64(p);
   rhi_out[i] = rhi;
   rlo_out[i] = rlo;
@@ -778,19 +833,29 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
   if (hipGetDeviceProperties(&e->prop, device) != hipSuccess) { delete e; return WF_E_DEVICE; }
   memset(&e->cfg, 0, sizeof e->cfg);
   if (cfg) e->cfg = *cfg;
-  if (!e->cfg.seg_bytes) e->cfg.seg_bytes = 32768;
-  if (!e->cfg.sieve_block) e->cfg.sieve_block = 512;
+  if (!e->cfg.seg_bytes) e->cfg.seg_bytes = 65536;
+  if (!e->cfg.sieve_block) e->cfg.sieve_block = 1024;
   if (!e->cfg.test_block) e->cfg.test_block = 256;
   if (!e->cfg.chunk) e->cfg.chunk = 10000000000ull;
-  if (!e->cfg.bucket_min) e->cfg.bucket_min = 65536;
-  if (e->cfg.seg_bytes < 4096 || (e->cfg.seg_bytes & (e->cfg.seg_bytes - 1)) ||
-      e->cfg.seg_bytes > 65536 || e->cfg.seg_bytes + 64 > e->prop.sharedMemPerBlock || e->cfg.sieve_block < 64 ||
+  if (!e->cfg.bucket_min) e->cfg.bucket_min = 262144;
+  if (!e->cfg.test_wgs) e->cfg.test_wgs = 16;
+  if (e->cfg.test_wgs > 16 || e->cfg.seg_bytes < 4096 || (e->cfg.seg_bytes & (e->cfg.seg_bytes - 1)) ||
+      e->cfg.seg_bytes > SEG_MAX || e->cfg.seg_bytes + 4096 > e->prop.sharedMemPerBlock || e->cfg.sieve_block < 64 ||
       e->cfg.sieve_block > 1024 || (e->cfg.sieve_block & 63) || e->cfg.test_block != 256 || e->cfg.arith > 1 ||
       e->cfg.bucket_min < 512 || e->cfg.chunk < 30ull * e->cfg.seg_bytes) {
     delete e;
     return WF_E_ARG;
   }
   e->sbshift = __builtin_ctz(e->cfg.seg_bytes);
+  // segments above 64 KiB need the kernel's dynamic LDS limit raised
+  if (hipFuncSetAttribute((const void*)k_sieve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)std::min<size_t>(e->prop.sharedMemPerBlock - 4096, SEG_MAX)) != hipSuccess) {
+    (void)hipGetLastError();
+    if (e->cfg.seg_bytes > 65536) {
+      delete e;
+      return WF_E_UNSUPPORTED;
+    }
+  }
   // both streams at the default priority
   if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking) != hipSuccess) {
@@ -823,7 +888,7 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
   // copy k (k = 0..3) holds the words starting at byte 4 i + k, long enough for any phase
   // plus the largest segment
   {
-    const u32 ext = PRE_PERIOD + 65536 + 8;
+    const u32 ext = PRE_PERIOD + SEG_MAX + 8;
     std::vector<unsigned char> pat(ext);
     for (u32 j = 0; j < ext; j++) {
       const u64 J = j % PRE_PERIOD;
@@ -834,7 +899,7 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
       }
       pat[j] = b;
     }
-    e->pre_words = (PRE_PERIOD + 65536) / 4 + 1;
+    e->pre_words = (PRE_PERIOD + SEG_MAX) / 4 + 1;
     std::vector<u32> w4(4 * (size_t)e->pre_words);
     for (u32 k = 0; k < 4; k++)
       for (u32 i = 0; i < e->pre_words; i++) {
@@ -892,9 +957,9 @@ extern "C" void wf_close(wf_engine* e) {
 
 extern "C" int wf_info(wf_engine* e, char* buf, int len) {
   if (!e || !buf || len <= 0) return WF_E_ARG;
-  snprintf(buf, (size_t)len, "%s (%s), %d CUs, LDS segment %u B, %s", e->prop.name, e->prop.gcnArchName,
-           e->prop.multiProcessorCount, e->cfg.seg_bytes,
-           e->cfg.arith == 1 ? "32-bit Montgomery" : "FP64 p-adic below 2^50, 32-bit Montgomery above");
+  snprintf(buf, (size_t)len, "%s (%s), %d CUs, LDS segment %u B (%zu B per workgroup), %s", e->prop.name,
+           e->prop.gcnArchName, e->prop.multiProcessorCount, e->cfg.seg_bytes, (size_t)e->prop.sharedMemPerBlock,
+           e->cfg.arith == 1 ? "32-bit Montgomery" : "FP64 p-adic below 2^50, radix-2^28 Montgomery above");
   return WF_OK;
 }
 
@@ -917,7 +982,7 @@ static void split_classes(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vect
   if (e->cfg.arith == 1) {
     cuts = {{1ull << 31, A_M2}, {1ull << 47, A_M3}, {1ull << 63, A_M4}, {0, A_FULL}};
   } else {
-    cuts = {{1ull << 50, A_F64}, {1ull << 63, A_M4}, {0, A_FULL}};
+    cuts = {{1ull << 50, A_F64}, {0, A_M28}};
   }
   u64 cur = lo;
   for (auto& c : cuts) {
@@ -1066,7 +1131,7 @@ static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, 
   if (P.bcap) {
     hipLaunchKernelGGL(k_zero, dim3((u32)std::min<u64>((nseg + 255) / 256, 1024)), dim3(256), 0, st, B.bcnt, nseg);
     if (!P.fills.empty()) {
-      hipLaunchKernelGGL(k_bucket_fill, dim3((u32)P.fills.size()), dim3(256), 0, st, dfills, e->d_sp,
+      hipLaunchKernelGGL(k_bucket_fill, dim3((u32)P.fills.size()), dim3(FILL_BLOCK), 0, st, dfills, e->d_sp,
                          e->d_spinv, B.bcnt, B.bdata, P.bcap, e->sbshift, e->d_flags);
       HIPCHK(hipGetLastError());
     }
@@ -1080,10 +1145,12 @@ static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, 
 }
 
 // Fermat-quotient tests of pass P's primes (buffer set B) on stream st.
+// wgs = test workgroups per CU: a full machine (16) when the tests run alone, fewer in the
+// pipelined path so that the next pass's sieve finds room beside them.
 static int enqueue_test(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, BufSet& B, hipStream_t st, int arith,
-                        const TestOut& to) {
+                        const TestOut& to, u32 wgs = 16) {
   const u64 nseg = P.segs.size();
-  const u32 tgrid = (u32)std::min<u64>(nseg * P.tps, (u64)e->prop.multiProcessorCount * 16);
+  const u32 tgrid = (u32)std::min<u64>(nseg * P.tps, (u64)e->prop.multiProcessorCount * wgs);
 #define WF_TEST(A) hipLaunchKernelGGL(k_test<A>, dim3(tgrid), dim3(256), 0, st, dsegs, (u32)nseg, B.off, B.cnt, \
                                       P.pcap, P.tps, to)
   switch (arith) {
@@ -1091,7 +1158,8 @@ static int enqueue_test(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, B
     case A_M2: WF_TEST(A_M2); break;
     case A_M3: WF_TEST(A_M3); break;
     case A_M4: WF_TEST(A_M4); break;
-    default: WF_TEST(A_FULL); break;
+    case A_FULL: WF_TEST(A_FULL); break;
+    default: WF_TEST(A_M28); break;
   }
 #undef WF_TEST
   HIPCHK(hipGetLastError());
@@ -1201,7 +1269,9 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
       rc = WF_E_DEVICE;
       break;
     }
-    if ((rc = enqueue_test(e, P, e->d_segs + soff[k], e->bs[set], e->st2, passes[k].second, to)) != WF_OK) break;
+    if ((rc = enqueue_test(e, P, e->d_segs + soff[k], e->bs[set], e->st2, passes[k].second, to, e->cfg.test_wgs)) !=
+        WF_OK)
+      break;
     if (hipEventRecord(tested, e->st2) != hipSuccess) { rc = WF_E_DEVICE; break; }
   }
   // join the test stream back into the capturing stream
@@ -1276,7 +1346,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   const u64 segspan = 30ull * e->cfg.seg_bytes;
   const u64 step = std::max<u64>(1, (u64)(chunk / (double)segspan)) * segspan;
   std::vector<std::pair<std::vector<Piece>, int>> passes;
-  for (int cls = A_F64; cls <= A_FULL; cls++) {
+  for (int cls = A_F64; cls < A_NCLS; cls++) {
     std::vector<Piece> batch;
     double in_batch = 0;
     for (const Piece& p0 : all) {
@@ -1381,7 +1451,8 @@ extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* 
   int rc = WF_OK;
   if (hipMemcpyAsync(d, p, n * sizeof(u64), hipMemcpyHostToDevice, e->st) != hipSuccess) rc = WF_E_DEVICE;
   if (rc == WF_OK) {
-    hipLaunchKernelGGL(k_quot, dim3((u32)((n + 255) / 256)), dim3(256), 0, e->st, d, d + n, d + 2 * n, d + 3 * n, n);
+    hipLaunchKernelGGL(k_quot, dim3((u32)((n + 255) / 256)), dim3(256), 0, e->st, d, d + n, d + 2 * n, d + 3 * n, n,
+                       e->cfg.arith == 1 ? 1 : 0);
     if (hipGetLastError() != hipSuccess) rc = WF_E_DEVICE;
   }
   if (rc == WF_OK && (hipMemcpyAsync(q, d + n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st) != hipSuccess ||
@@ -1463,12 +1534,12 @@ extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double
   const wf_config orig = e->cfg;
   wf_config bestc = orig;
   double bestt = 1e300;
-  const u32 segs[] = {16384, 32768, 65536}, blocks[] = {256, 512, 1024},
+  const u32 segs[] = {16384, 32768, 65536, 131072}, blocks[] = {256, 512, 1024},
             bmins[] = {32768, 65536, 131072, 262144, 524288, 1048576};
   for (u32 sb : segs)
     for (u32 bl : blocks)
       for (u32 bm : bmins) {
-        if (sb + 64 > e->prop.sharedMemPerBlock) continue;
+        if (sb + 4096 > e->prop.sharedMemPerBlock) continue;
         wf_config c = orig;
         c.seg_bytes = sb;
         c.sieve_block = bl;
@@ -1574,7 +1645,7 @@ static void serve_loop(wf_engine* e) {
     to.ev_cap = e->dev_ev_cap;
     to.nev = e->d_ctr + 1;
     to.near_T = 0;
-    for (int cls = A_F64; cls <= A_FULL && rc == WF_OK; cls++) {
+    for (int cls = A_F64; cls < A_NCLS && rc == WF_OK; cls++) {
       std::vector<Piece> ps;
       for (const Piece& p : all)
         if (p.arith == cls) ps.push_back(p);

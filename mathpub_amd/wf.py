@@ -30,7 +30,7 @@ class Range(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("seg_bytes", u32), ("sieve_block", u32), ("test_block", u32), ("arith", u32), ("bucket_min", u32),
-                ("chunk", u64)]
+                ("chunk", u64), ("test_wgs", u32)]
 
 
 class Timing(ctypes.Structure):
@@ -104,9 +104,9 @@ def result_dict(res, events=None):
 class Engine:
     """One GPU engine (wf_open); not thread-safe, like the C object."""
 
-    def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0, bucket_min=0):
+    def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0, bucket_min=0, test_wgs=0):
         L = load()
-        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk)
+        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs)
         h = ctypes.c_void_p()
         _check(L.wf_open(device, ctypes.byref(cfg), ctypes.byref(h)), "wf_open")
         self._h = h

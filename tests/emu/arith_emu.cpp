@@ -32,4 +32,11 @@ void emu_mont(int L, int full, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo
 
 int emu_sprp2(uint64_t n) { return sprp2_64(n) ? 1 : 0; }
 
+void emu_m28(uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
+  uint32_t x[4];
+  m28_pow2(p, p - 1, ebits, x);
+  *lo = (uint64_t)x[0] | ((uint64_t)x[1] << 32);
+  *hi = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
+}
+
 }

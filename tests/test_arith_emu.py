@@ -24,6 +24,7 @@ def emu():
     L.emu_fp64.argtypes = [ctypes.c_uint64, P, P]
     L.emu_mont.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, P, P]
     L.emu_sprp2.argtypes = [ctypes.c_uint64]
+    L.emu_m28.argtypes = [ctypes.c_uint64, ctypes.c_int, P, P]
     return L
 
 
@@ -93,3 +94,14 @@ def test_sprp2_64(emu):
         if n < 3:
             continue
         assert bool(emu.emu_sprp2(n)) == strong2(n), n
+
+
+def test_radix28_montgomery(emu):
+    ps = samples(47, 64, 8000, 99) + [(1 << 47) + 1, (1 << 50) - 27, (1 << 63) - 25, (1 << 63) + 29,
+                                      (1 << 64) - 59, (1 << 64) - 1, (1 << 64) - 3, 10 ** 18 + 9]
+    for p in ps:
+        if p < (1 << 47):
+            continue
+        eb = (p - 1).bit_length()
+        assert call(emu.emu_m28, p, eb) == pow(2, p - 1, p * p), p
+        assert call(emu.emu_m28, p, min(64, eb + 2)) == pow(2, p - 1, p * p), p

@@ -124,11 +124,30 @@ def test_autotune_keeps_results(eng):
     before = eng.scan(lo, hi)
     cfg = eng.autotune(lo, hi)
     say("autotune", cfg)
-    assert cfg["seg_bytes"] in (16384, 32768, 65536)
+    assert cfg["seg_bytes"] in (16384, 32768, 65536, 131072)
     after = eng.scan(lo, hi)
     per, ev = eng.scan_batch([(lo, hi)])
     for x in (after, per[0]):
         assert (x["primes"], x["cks"], x["sumq"]) == (before["primes"], before["cks"], before["sumq"])
+
+
+@pytest.mark.parametrize("seg,block,bmin", [(16384, 256, 512), (65536, 1024, 262144), (131072, 1024, 524288),
+                                            (131072, 512, 32768)])
+def test_segment_configs(seg, block, bmin):
+    """Every LDS segment size, sieve workgroup and bucket threshold gives the same results."""
+    e = wf.Engine(seg_bytes=seg, sieve_block=block, bucket_min=bmin)
+    try:
+        for lo, hi, psp in [(3, 2 * 10 ** 7, False), (10 ** 12 + 12345, 10 ** 12 + 2 * 10 ** 8 + 7, False),
+                            ((1 << 64) - 10 ** 7, 1 << 64, True)]:
+            say("segment config", seg, block, bmin, lo, hi)
+            r = e.scan(lo, hi, near_T=1000, psp=psp)
+            o = oracle.scan(lo, hi, bases=[2], near_T=1000, psp=psp)
+            assert (r["primes"], r["tests"], r["near"]) == (o["primes"], o["tests"], o["near"])
+            assert r["cks"] == o["cks"][2] and r["sumq"] == o["sumq"][2]
+            if psp:
+                assert r["psp"] == o["psp"]
+    finally:
+        e.close()
 
 
 def test_serving_same_alone_and_in_mix():
