@@ -310,6 +310,7 @@ def main():
         roof["w_products_per_s"] = tests * wf_products(pbits) / (test_ms / 1e3)
         roof["sieve_ms_per_step"] = sieve_ms
         roof["test_ms_per_step"] = test_ms
+        roof["host_plan_ms_per_step"] = sum(t["plan_ms"] for t in ptim) / args.steps
     cfg = {"workload": "single range [1e15, 1e15+%.3g), base 2, near |A|<=%d" % (span, NEAR_T),
            "engine": "batched" if batched else "plain", "primes_per_step": r["primes"], "tests_per_step": tests,
            "cks": "%016x" % r["cks"], "sumq": "%016x" % r["sumq"], "near": r["near"],
@@ -320,6 +321,8 @@ def main():
         cfg["plain_value"] = pv
         cfg["plain_ms_per_step"] = plain[2]
         cfg["batched_over_plain"] = value / pv
+    if batched:
+        cfg["batched_host_plan_ms_per_step"] = sum(t["plan_ms"] for t in batched[1]) / args.steps
     out = {
         "metric": METRIC, "value": value, "unit": "primes tested/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",

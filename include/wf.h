@@ -73,7 +73,7 @@ typedef struct { uint64_t lo, hi; } wf_range;   /* [lo, hi), hi == 0 means 2^64 
 
 /* Tuning knobs; a zero field means "engine default". */
 typedef struct {
-    uint32_t seg_bytes;     /* LDS sieve segment, 16384 .. 65536 (a power of two) */
+    uint32_t seg_bytes;     /* LDS sieve segment, 4096 .. 131072 (a power of two) */
     uint32_t sieve_block;   /* threads per sieve workgroup (256 .. 1024) */
     uint32_t test_block;    /* threads per test workgroup (64 .. 1024) */
     uint32_t arith;         /* 0 = fastest exact path, 1 = force 32-bit Montgomery */
@@ -122,15 +122,16 @@ int  wf_serve_submit(wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *ticket);
 int  wf_serve_poll(wf_engine *e, uint64_t ticket, wf_result *out, int *done);
 int  wf_serve_stop(wf_engine *e);
 
-/* Autotuner: times the sieve stage on a sample of [lo, hi) over segment sizes 16-64 KiB,
- * sieve workgroups of 4-16 wavefronts and bucket thresholds 2^15-2^18, switches the engine
+/* Autotuner: times the sieve stage on a sample of [lo, hi) over segment sizes 16-128 KiB,
+ * sieve workgroups of 4-16 wavefronts and bucket thresholds 2^15-2^20, switches the engine
  * to the fastest and returns it (*best_ms = its sieve milliseconds per 1e9 numbers).
  * No counterpart in the reference (its chunk and thread counts are set by hand). */
 int  wf_autotune(wf_engine *e, uint64_t lo, uint64_t hi, wf_config *best, double *best_ms);
 
 /* Timings of the last wf_scan / wf_scan_batch: device milliseconds of the sieve and
- * the test kernels (HIP events on the engine's stream), primes tested. */
-typedef struct { double sieve_ms, test_ms, total_ms; uint64_t tested; uint64_t launches; } wf_timing;
+ * the test kernels (HIP events on the engine's stream), primes tested, device passes, and
+ * the host milliseconds spent planning the passes (segments and fill work items). */
+typedef struct { double sieve_ms, test_ms, total_ms; uint64_t tested; uint64_t launches; double plan_ms; } wf_timing;
 int  wf_last_timing(wf_engine *e, wf_timing *t);
 
 #ifdef __cplusplus

@@ -16,11 +16,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <condition_variable>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -123,8 +125,10 @@ __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c, const Wh
 #define SEG_MAX 131072    // largest LDS segment in bytes (a bucket record is byte << 3 | class)
 #define PRE_PERIOD 323323u  // 7 * 11 * 13 * 17 * 19: period of the presieve pattern in wheel bytes
 #define PRE_COUNT 5u        // the presieve primes are the first five sieving primes
-#define FILL_STAGE 8192   // records a fill workgroup stages in LDS (32 KiB)
-#define FILL_TARGET 4096  // expected records per fill work item (the host sizes items to it)
+#ifndef FILL_STAGE
+#define FILL_STAGE 16384  // records a fill workgroup stages in LDS (64 KiB)
+#endif
+#define FILL_TARGET (FILL_STAGE / 2)  // expected records per fill work item (the host sizes items to it)
 #ifndef FILL_BLOCK
 #define FILL_BLOCK 512    // threads per fill workgroup
 #endif
@@ -680,6 +684,9 @@ struct wf_engine {
   // sieving primes (odd primes >= 7, ascending) and prefix sums of 1/q
   std::vector<u32> h_sp;
   std::vector<double> h_inv_sum;
+  // fill slices: (first prime index of a [Q, 2Q) group, segments per run) -> the slice
+  // ends over the whole group; cleared whenever the prime table or the configuration changes
+  std::map<std::pair<u32, u32>, std::vector<u32>> slices;
   u64 sp_lim = 0;
   u32* d_sp = nullptr;
   u64* d_spinv = nullptr;
@@ -805,6 +812,7 @@ static int ensure_sieving_primes(wf_engine* e, u64 lim) {
   e->q1idx = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), e->cfg.bucket_min) - e->h_sp.begin());
   e->h_inv_sum.assign(e->h_sp.size() + 1, 0.0);
   for (size_t i = 0; i < e->h_sp.size(); i++) e->h_inv_sum[i + 1] = e->h_inv_sum[i] + 1.0 / e->h_sp[i];
+  e->slices.clear();
   return WF_OK;
 }
 
@@ -1054,7 +1062,29 @@ struct PassPlan {
   u32 maxnq = 0, bcap = 0, pcap = 0, tps = 0;
 };
 
-static void plan_pass(const wf_engine* e, const std::vector<Piece>& pieces, PassPlan& P) {
+// Slice ends for the primes [i, gend) of one [Q, 2Q) group over runs of ns segments: each
+// slice holds about FILL_TARGET expected records (prime q leaves 8 seg_bytes ns / q of them),
+// and at most 65536 primes.  Cached per (group, ns): passes of one range repeat them.
+static const std::vector<u32>& fill_slices(wf_engine* e, u32 i, u32 gend, u32 ns) {
+  auto key = std::make_pair(i, ns);
+  auto it = e->slices.find(key);
+  if (it != e->slices.end()) return it->second;
+  std::vector<u32> ends;
+  const double per = 8.0 * e->cfg.seg_bytes * ns;
+  u32 a = i;
+  while (a < gend) {
+    const double lim = e->h_inv_sum[a] + FILL_TARGET / per;
+    u32 b = (u32)(std::upper_bound(e->h_inv_sum.begin() + a + 1, e->h_inv_sum.begin() + gend + 1, lim) -
+                  e->h_inv_sum.begin()) - 1;
+    if (b <= a) b = a + 1;
+    b = std::min(b, std::min<u32>(gend, a + 65536));
+    ends.push_back(b);
+    a = b;
+  }
+  return e->slices.emplace(key, std::move(ends)).first->second;
+}
+
+static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& P) {
   P.segs.clear();
   P.runs.clear();
   for (const Piece& p : pieces) build_segments(e, p.lo, p.hi, p.rid, P.segs, P.runs);
@@ -1080,20 +1110,16 @@ static void plan_pass(const wf_engine* e, const std::vector<Piece>& pieces, Pass
       while (i < ch.nq) {
         const u32 Q = e->h_sp[i];
         const u32 qend = Q >= 0x80000000u ? 0xFFFFFFFFu : 2 * Q;
-        const u32 bend = std::min<u32>(ch.nq, (u32)(std::lower_bound(e->h_sp.begin() + i, e->h_sp.end(), qend) -
-                                                     e->h_sp.begin()));
+        const u32 gend = (u32)(std::lower_bound(e->h_sp.begin() + i, e->h_sp.end(), qend) - e->h_sp.begin());
+        const u32 bend = std::min<u32>(ch.nq, gend);
         u32 G = 1;
         while (G < RUN_MAX && 2.0 * G * 8.0 * SBd <= 64.0 * Q) G *= 2;
         for (u32 s0 = 0; s0 < ch.nseg; s0 += G) {
           const u32 ns = std::min(G, ch.nseg - s0);
-          const double per = 8.0 * SBd * ns;     // expected records of prime q: per / q
+          const std::vector<u32>& ends = fill_slices(e, i, gend, ns);
           u32 a = i;
-          while (a < bend) {
-            const double lim = e->h_inv_sum[a] + FILL_TARGET / per;
-            u32 b = (u32)(std::upper_bound(e->h_inv_sum.begin() + a + 1, e->h_inv_sum.begin() + bend + 1, lim) -
-                          e->h_inv_sum.begin()) - 1;
-            if (b <= a) b = a + 1;
-            b = std::min(b, std::min<u32>(bend, a + 65536));
+          for (size_t k = 0; k < ends.size() && a < bend; k++) {
+            const u32 b = std::min(ends[k], bend);
             FillDesc f;
             f.base = ch.base + (u64)s0 * 30ull * e->cfg.seg_bytes;
             f.seg0 = ch.seg0 + s0;
@@ -1104,7 +1130,7 @@ static void plan_pass(const wf_engine* e, const std::vector<Piece>& pieces, Pass
             a = b;
           }
         }
-        i = bend;
+        i = gend >= ch.nq ? ch.nq : gend;
       }
     }
   }
@@ -1181,10 +1207,16 @@ static int upload_pass(wf_engine* e, const PassPlan& P) {
 }
 
 // Plain path: sieve and test one pass on the engine's stream, then wait for it.
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith, const TestOut& to,
                       bool want_psp = false) {
   PassPlan P;
+  const auto t0 = std::chrono::steady_clock::now();
   plan_pass(e, pieces, P);
+  e->last.plan_ms += ms_since(t0);
   if (P.segs.empty()) return WF_OK;
   int rc;
   if ((rc = size_set(e->bs[0], P)) != WF_OK) return rc;
@@ -1225,6 +1257,7 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
   std::vector<PassPlan> plans(passes.size());
   std::vector<u64> soff(passes.size()), roff(passes.size()), foff(passes.size());
   u64 nsegs = 0, nruns = 0, nfills = 0;
+  const auto t0 = std::chrono::steady_clock::now();
   for (size_t k = 0; k < passes.size(); k++) {
     plan_pass(e, passes[k].first, plans[k]);
     soff[k] = nsegs;
@@ -1234,6 +1267,7 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
     nruns += plans[k].runs.size();
     nfills += plans[k].fills.size();
   }
+  e->last.plan_ms += ms_since(t0);
   if (nsegs == 0) return WF_OK;
   int rc;
   if ((rc = grow(e->d_segs, e->seg_cap, nsegs)) != WF_OK) return rc;
@@ -1516,11 +1550,12 @@ static void apply_config(wf_engine* e, const wf_config& c) {
   e->sbshift = __builtin_ctz(e->cfg.seg_bytes);
   if (!e->h_sp.empty())
     e->q1idx = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), e->cfg.bucket_min) - e->h_sp.begin());
+  e->slices.clear();
 }
 
 // Autotuner: time the sieve stage (bucket fill + segment sieve + extraction) of a sample
-// of [lo, hi) for every segment size 16-64 KiB, sieve workgroup of 4-16 wavefronts and
-// bucket threshold 2^15-2^18, keep the fastest.  The test kernel does not depend on them.
+// of [lo, hi) for every segment size 16-128 KiB, sieve workgroup of 4-16 wavefronts and
+// bucket threshold 2^15-2^20, keep the fastest.  The test kernel does not depend on them.
 extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double* best_ms) {
   if (!e || (hi != 0 && hi <= lo)) return WF_E_ARG;
   HIPCHK(hipSetDevice(e->dev));

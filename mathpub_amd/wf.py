@@ -35,7 +35,7 @@ class Config(ctypes.Structure):
 
 class Timing(ctypes.Structure):
     _fields_ = [("sieve_ms", ctypes.c_double), ("test_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("tested", u64), ("launches", u64)]
+                ("tested", u64), ("launches", u64), ("plan_ms", ctypes.c_double)]
 
 
 class WFError(RuntimeError):
@@ -206,4 +206,4 @@ class Engine:
         t = Timing()
         _check(self._L.wf_last_timing(self._h, ctypes.byref(t)), "wf_last_timing")
         return {"sieve_ms": t.sieve_ms, "test_ms": t.test_ms, "total_ms": t.total_ms, "tested": t.tested,
-                "launches": t.launches}
+                "launches": t.launches, "plan_ms": t.plan_ms}
