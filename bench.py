@@ -112,8 +112,10 @@ def bench_c3(args):
         per, ev = eng.scan_batch(ranges)
         print("c3 warmup %d: %.2f s" % (i, time.perf_counter() - t), file=sys.stderr, flush=True)
     t0 = time.perf_counter()
+    tims = []
     for _ in range(args.steps):
         per, ev = eng.scan_batch(ranges)
+        tims.append(eng.last_timing())
     ms = (time.perf_counter() - t0) / args.steps * 1e3
     tests = sum(r["tests"] for r in per)
     numbers = sum(h - l for l, h in ranges)
@@ -126,7 +128,9 @@ def bench_c3(args):
                         ms, args, {"workload": "4096 ranges, starts U[1e12,1e18], lengths logU[1e6,1e9], seed "
                                    "0x243F6A8885A308D3", "engine": "batched", "ranges": len(ranges),
                                    "numbers": numbers, "tests": tests, "per_range_digest": "%016x" % digest,
-                                   "autotuned": tuned, "w_products_per_s": prods / (ms / 1e3)})
+                                   "autotuned": tuned, "w_products_per_s": prods / (ms / 1e3),
+                                   "device_ms_per_step": sum(t["total_ms"] for t in tims) / len(tims),
+                                   "host_plan_ms_per_step": sum(t["plan_ms"] for t in tims) / len(tims)})
 
 
 def bench_c4(args):

@@ -84,17 +84,24 @@ __constant__ u32 c_R30[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 __constant__ unsigned char c_M30[8 * 30 * 8];
 // c_QI[q mod 30] = index of q mod 30 in R30 (q coprime to 30)
 __constant__ unsigned char c_QI[30];
+// c_W30[r] = index of r in R30, -1 if r shares a factor with 30
+__constant__ signed char c_W30[30] = {-1, 0, -1, -1, -1, -1, -1, 1, -1, -1, -1, 2, -1, 3, -1,
+                                      -1, -1, 4, -1, 5, -1, -1, -1, 6, -1, -1, -1, -1, -1, 7};
 
 // The two wheel tables, copied to LDS by every workgroup that looks them up per lane
 // (per-lane lookups in constant memory are vector memory loads with cache latency).
 struct WheelLds {
   unsigned char m30[8 * 30 * 8];
   unsigned char qi[32];
+  signed char w30[32];
 };
 
 __device__ __forceinline__ void load_wheel(WheelLds& w) {
   for (u32 k = threadIdx.x; k < 8 * 30 * 8; k += blockDim.x) w.m30[k] = c_M30[k];
-  if (threadIdx.x < 30) w.qi[threadIdx.x] = c_QI[threadIdx.x];
+  if (threadIdx.x < 30) {
+    w.qi[threadIdx.x] = c_QI[threadIdx.x];
+    w.w30[threadIdx.x] = c_W30[threadIdx.x];
+  }
 }
 
 // Byte index j (relative to the segment base) of the first mark of class c for a prime q
@@ -108,6 +115,19 @@ __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c, const Wh
   const u64 v = (u64)m * q + o32 - c_R30[c];
   if (v >= (30ull << 32)) return ~0ull;   // beyond any segment or run (q close to 2^32)
   return (u64)((u32)(v >> 1) * 0xEEEEEEEFu);
+}
+
+// The multiples x = o, o + q, ... below span (offsets from a base divisible by 30, span
+// < 2^32) of a prime q: f(byte, class) for each one on the wheel.  For q > span / 8 (a few
+// multiples at most) this costs less than finding the eight class starts.
+template <class F>
+__device__ __forceinline__ void walk_multiples(u64 o, u32 q, u64 span, const WheelLds& w, F f) {
+  for (u64 x = o; x < span; x += q) {
+    const u32 xb = (u32)x;
+    const u32 byte = xb / 30u;
+    const int c = w.w30[xb - byte * 30u];
+    if (c >= 0) f(byte, c);
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -147,11 +167,16 @@ struct SegDesc {
 };
 
 // consecutive segments seg0 .. seg0 + nseg - 1 of one range piece, base = seg0's base
+// (host side: the unit the fill groups are cut from)
 struct RunDesc { u64 base; u32 seg0, nseg, nq, pad; };
 
-// one k_bucket_fill work item: sieving primes [i0, i1) over the nseg segments from seg0
-// (their numbers start at base)
-struct FillDesc { u64 base; u32 seg0, nseg, i0, i1; };
+// consecutive segments seg0 .. seg0 + nseg - 1 of one range piece (numbers from base),
+// the segments lseg0 .. lseg0 + nseg - 1 of its fill group
+struct FillRun { u64 base; u32 seg0, nseg, lseg0, pad; };
+
+// one k_bucket_fill work item: sieving primes [i0, i1) over the fill runs fr0 .. fr0 + nfr - 1,
+// nseg segments in all (<= RUN_MAX)
+struct FillDesc { u32 fr0, nfr, i0, i1, nseg, pad; };
 
 struct DevAcc {     // per range; every field is a 64-bit sum
   unsigned long long primes, tests, near, flt, cks, sumq, nev, npsp;
@@ -172,27 +197,34 @@ __device__ __forceinline__ bool first_offset(u64 base, u32 q, u64 minv, u64 span
   return o < span;
 }
 
-// Large primes -> bucket records of a run's segments.  A work item covers a slice of the
-// large primes over one run of <= RUN_MAX segments, sized by the host to about FILL_TARGET
-// expected records.  Walk 1 counts the hits per segment in LDS; one global atomic per
-// segment reserves room in its bucket; walk 2 stages the records in LDS grouped by
-// segment; the workgroup then copies each group to its bucket with consecutive lanes on
-// consecutive addresses.  Records that do not fit the stage go straight to the bucket.
+// Large primes -> bucket records.  A work item covers a slice of the large primes over one
+// fill group: consecutive chunks of segments (of one range, or of several short ranges),
+// RUN_MAX segments at most, sized by the host to about FILL_TARGET expected records.  A
+// prime's table entry is read once per group, and one Barrett reduction per run places it.
+// Walk 1 counts the hits per segment in LDS; one global atomic per segment reserves room
+// in its bucket; walk 2 stages the records in LDS grouped by segment; the workgroup then
+// copies each group to its bucket with consecutive lanes on consecutive addresses.
+// Records that do not fit the stage go straight to the bucket.
 __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __restrict__ items,
+                                                     const FillRun* __restrict__ fruns,
                                                      const u32* __restrict__ sp, const u64* __restrict__ spinv,
                                                      u32* __restrict__ bcnt, u32* __restrict__ bdata,
                                                      u32 bcap, int sbshift, u32* __restrict__ flags) {
-  __shared__ u32 hist[RUN_MAX], loff[RUN_MAX + 1], gbase[RUN_MAX];
+  __shared__ u32 cur[RUN_MAX];    // hits per segment, then the staging cursor
+  __shared__ u32 dloc[RUN_MAX];   // bucket slot of a segment's record = dloc + stage slot
+  __shared__ u32 gseg[RUN_MAX];   // group segment -> segment of the pass
   __shared__ u32 stage[FILL_STAGE];
   __shared__ u32 wtot[FILL_BLOCK / 64];
+  __shared__ u32 total;
   __shared__ WheelLds wl;
   load_wheel(wl);
   const FillDesc it = items[blockIdx.x];
-  const FillDesc& r = it;
-  const u64 nbytes = (u64)r.nseg << sbshift;
-  const u64 span = 30ull * nbytes;
   const u32 smask = (1u << sbshift) - 1;
-  for (u32 k = threadIdx.x; k < r.nseg; k += blockDim.x) hist[k] = 0;
+  for (u32 k = threadIdx.x; k < it.nseg; k += blockDim.x) cur[k] = 0;
+  for (u32 r = 0; r < it.nfr; r++) {
+    const FillRun fr = fruns[it.fr0 + r];
+    for (u32 k = threadIdx.x; k < fr.nseg; k += blockDim.x) gseg[fr.lseg0 + k] = fr.seg0 + k;
+  }
   __syncthreads();
   // few primes with many hits each: one lane per (prime, class); else one lane per prime
   const bool pairs = it.i1 - it.i0 < blockDim.x;
@@ -200,71 +232,89 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
   for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
     const u32 i = it.i0 + (pairs ? w >> 3 : w);
     const u32 q = sp[i];
-    u64 o;
-    if (!first_offset(r.base, q, spinv[i], span, o)) continue;
+    const u64 minv = spinv[i];
     const int qi = wl.qi[q % 30];
     const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
-    for (int c = c0; c < c1; c++)
-      for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q) atomicAdd(&hist[j >> sbshift], 1u);
+    for (u32 r = 0; r < it.nfr; r++) {
+      const FillRun fr = fruns[it.fr0 + r];
+      const u64 nbytes = (u64)fr.nseg << sbshift;
+      const u64 span = 30ull * nbytes;
+      u64 o;
+      if (!first_offset(fr.base, q, minv, span, o)) continue;
+      if (!pairs && 8ull * q > span) {
+        walk_multiples(o, q, span, wl, [&](u32 j, int) { atomicAdd(&cur[fr.lseg0 + (j >> sbshift)], 1u); });
+        continue;
+      }
+      for (int c = c0; c < c1; c++)
+        for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q)
+          atomicAdd(&cur[fr.lseg0 + (u32)(j >> sbshift)], 1u);
+    }
   }
   __syncthreads();
-  // loff = exclusive prefix sums of hist: each thread scans a few consecutive segments,
-  // the wavefront and workgroup totals give its starting point
+  // exclusive prefix sums of the counts (each thread scans a few consecutive segments, the
+  // wavefront and workgroup totals give its start) and one bucket reservation per segment
   {
-    const u32 per = (r.nseg + blockDim.x - 1) / blockDim.x;
-    const u32 k0 = min(threadIdx.x * per, r.nseg), k1 = min(k0 + per, r.nseg);
+    const u32 per = (it.nseg + blockDim.x - 1) / blockDim.x;
+    const u32 k0 = min(threadIdx.x * per, it.nseg), k1 = min(k0 + per, it.nseg);
     u32 local = 0;
-    for (u32 k = k0; k < k1; k++) local += hist[k];
+    for (u32 k = k0; k < k1; k++) local += cur[k];
     const u32 incl = wave_incl_scan(local);
     if ((threadIdx.x & 63) == 63) wtot[threadIdx.x >> 6] = incl;
     __syncthreads();
     u32 acc = incl - local;
     for (u32 w = 0; w < (threadIdx.x >> 6); w++) acc += wtot[w];
     for (u32 k = k0; k < k1; k++) {
-      loff[k] = acc;
-      acc += hist[k];
+      const u32 h = cur[k];
+      u32 b = 0;
+      if (h) {
+        b = atomicAdd(&bcnt[gseg[k]], h);
+        if (b + h > bcap) atomicOr(flags, 1u);
+      }
+      dloc[k] = b - acc;   // mod 2^32
+      cur[k] = acc;
+      acc += h;
     }
-    if (threadIdx.x == blockDim.x - 1) loff[r.nseg] = acc;
-  }
-  __syncthreads();
-  for (u32 k = threadIdx.x; k < r.nseg; k += blockDim.x) {
-    const u32 h = hist[k];
-    u32 b = 0;
-    if (h) {
-      b = atomicAdd(&bcnt[r.seg0 + k], h);
-      if (b + h > bcap) atomicOr(flags, 1u);
-    }
-    gbase[k] = b;
-    hist[k] = loff[k];   // becomes the staging cursor
+    if (threadIdx.x == blockDim.x - 1) total = acc;
   }
   __syncthreads();
   for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
     const u32 i = it.i0 + (pairs ? w >> 3 : w);
     const u32 q = sp[i];
-    u64 o;
-    if (!first_offset(r.base, q, spinv[i], span, o)) continue;
+    const u64 minv = spinv[i];
     const int qi = wl.qi[q % 30];
     const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
-    for (int c = c0; c < c1; c++)
-      for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q) {
-        const u32 sg = (u32)(j >> sbshift);
-        const u32 rec = (((u32)j & smask) << 3) | (u32)c;
-        const u32 slot = atomicAdd(&hist[sg], 1u);
+    for (u32 r = 0; r < it.nfr; r++) {
+      const FillRun fr = fruns[it.fr0 + r];
+      const u64 nbytes = (u64)fr.nseg << sbshift;
+      const u64 span = 30ull * nbytes;
+      u64 o;
+      if (!first_offset(fr.base, q, minv, span, o)) continue;
+      auto put = [&](u32 j, int c) {
+        const u32 sg = fr.lseg0 + (j >> sbshift);
+        const u32 rec = ((j & smask) << 3) | (u32)c;
+        const u32 slot = atomicAdd(&cur[sg], 1u);
         if (slot < FILL_STAGE) {
           stage[slot] = (sg << 20) | rec;
         } else {
-          const u32 pos = gbase[sg] + (slot - loff[sg]);
-          if (pos < bcap) bdata[(u64)(r.seg0 + sg) * bcap + pos] = rec;
+          const u32 pos = dloc[sg] + slot;
+          if (pos < bcap) bdata[(u64)gseg[sg] * bcap + pos] = rec;
         }
+      };
+      if (!pairs && 8ull * q > span) {
+        walk_multiples(o, q, span, wl, put);
+        continue;
       }
+      for (int c = c0; c < c1; c++)
+        for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q) put((u32)j, c);
+    }
   }
   __syncthreads();
-  const u32 nst = min(loff[r.nseg], (u32)FILL_STAGE);
+  const u32 nst = min(total, (u32)FILL_STAGE);
   for (u32 k = threadIdx.x; k < nst; k += blockDim.x) {
     const u32 v = stage[k];
     const u32 sg = v >> 20;
-    const u32 pos = gbase[sg] + (k - loff[sg]);
-    if (pos < bcap) bdata[(u64)(r.seg0 + sg) * bcap + pos] = v & 0xFFFFFu;
+    const u32 pos = dloc[sg] + k;
+    if (pos < bcap) bdata[(u64)gseg[sg] * bcap + pos] = v & 0xFFFFFu;
   }
 }
 
@@ -342,6 +392,10 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
         }
         u64 o;
         if (!first_offset(d.base, q, minv, span, o)) continue;
+        if (8ull * q > span) {
+          walk_multiples(o, q, span, wl, [&](u32 j, int c) { atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c)); });
+          continue;
+        }
         const int qi = wl.qi[q % 30];
 #pragma unroll 1
         for (int c = 0; c < 8; c++) {
@@ -615,8 +669,7 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
 // from the sieve bitmap, and the odd multiples of 3 and 5) gets the exact SPRP-2 test;
 // the ones that pass are appended to out[] and counted per range.
 #define PSP_TILE 1024
-__constant__ signed char c_W30[30] = {-1, 0, -1, -1, -1, -1, -1, 1, -1, -1, -1, 2, -1, 3, -1,
-                                      -1, -1, 4, -1, 5, -1, -1, -1, 6, -1, -1, -1, -1, -1, 7};
+
 
 __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u32 nseg,
                                              const u32* __restrict__ bitmap, u32 bmw, u32 tps,
@@ -694,7 +747,7 @@ struct wf_engine {
   // pass descriptors (all passes of a batch) and two sets of pass buffers
   SegDesc* d_segs = nullptr;
   u64 seg_cap = 0;
-  RunDesc* d_runs = nullptr;
+  FillRun* d_runs = nullptr;     // fill runs of the pass(es)
   u64 run_cap = 0;
   FillDesc* d_fills = nullptr;
   u64 fill_cap = 0;
@@ -1058,6 +1111,7 @@ static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vec
 struct PassPlan {
   std::vector<SegDesc> segs;
   std::vector<RunDesc> runs;
+  std::vector<FillRun> fruns;
   std::vector<FillDesc> fills;
   u32 maxnq = 0, bcap = 0, pcap = 0, tps = 0;
 };
@@ -1087,6 +1141,7 @@ static const std::vector<u32>& fill_slices(wf_engine* e, u32 i, u32 gend, u32 ns
 static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& P) {
   P.segs.clear();
   P.runs.clear();
+  P.fruns.clear();
   for (const Piece& p : pieces) build_segments(e, p.lo, p.hi, p.rid, P.segs, P.runs);
   P.maxnq = 0;
   for (const RunDesc& r : P.runs) P.maxnq = std::max(P.maxnq, r.nq);
@@ -1103,35 +1158,59 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
     // (<= RUN_MAX) nearest 64 Q / (8 seg_bytes): about 64 records per prime and run, so
     // the per-run start of a prime is paid for by its hits.  Each run's primes are cut
     // into slices of ~FILL_TARGET expected records.
+    // fill groups.  Primes in [Q, 2Q) use groups of at most G segments, G the power of two
+    // (<= RUN_MAX) nearest 64 Q / (8 seg_bytes): about 64 records per prime and group, so a
+    // prime's start in a group is paid for by its hits.  A group is a run of one range cut
+    // to G segments, or several short runs (of short ranges) packed together.  Each
+    // group's primes are cut into slices of ~FILL_TARGET expected records.
     const double SBd = e->cfg.seg_bytes;
-    for (const RunDesc& ch : P.runs) {
-      if (ch.nq <= e->q1idx) continue;
-      u32 i = e->q1idx;
-      while (i < ch.nq) {
-        const u32 Q = e->h_sp[i];
-        const u32 qend = Q >= 0x80000000u ? 0xFFFFFFFFu : 2 * Q;
-        const u32 gend = (u32)(std::lower_bound(e->h_sp.begin() + i, e->h_sp.end(), qend) - e->h_sp.begin());
-        const u32 bend = std::min<u32>(ch.nq, gend);
-        u32 G = 1;
-        while (G < RUN_MAX && 2.0 * G * 8.0 * SBd <= 64.0 * Q) G *= 2;
+    u32 i = e->q1idx;
+    while (i < P.maxnq) {
+      const u32 Q = e->h_sp[i];
+      const u32 qend = Q >= 0x80000000u ? 0xFFFFFFFFu : 2 * Q;
+      const u32 gend = (u32)(std::lower_bound(e->h_sp.begin() + i, e->h_sp.end(), qend) - e->h_sp.begin());
+      u32 G = 1;
+      while (G < RUN_MAX && 2.0 * G * 8.0 * SBd <= 64.0 * Q) G *= 2;
+      u32 g_fr0 = (u32)P.fruns.size(), g_nseg = 0, g_nq = 0;
+      auto flush = [&]() {
+        if (!g_nseg) return;
+        const u32 bend = std::min(gend, g_nq);
+        const std::vector<u32>& ends = fill_slices(e, i, gend, g_nseg);
+        u32 a = i;
+        for (size_t k = 0; k < ends.size() && a < bend; k++) {
+          const u32 b = std::min(ends[k], bend);
+          FillDesc f;
+          f.fr0 = g_fr0;
+          f.nfr = (u32)P.fruns.size() - g_fr0;
+          f.i0 = a;
+          f.i1 = b;
+          f.nseg = g_nseg;
+          f.pad = 0;
+          P.fills.push_back(f);
+          a = b;
+        }
+        g_fr0 = (u32)P.fruns.size();
+        g_nseg = 0;
+        g_nq = 0;
+      };
+      for (const RunDesc& ch : P.runs) {
+        if (ch.nq <= i) continue;
         for (u32 s0 = 0; s0 < ch.nseg; s0 += G) {
           const u32 ns = std::min(G, ch.nseg - s0);
-          const std::vector<u32>& ends = fill_slices(e, i, gend, ns);
-          u32 a = i;
-          for (size_t k = 0; k < ends.size() && a < bend; k++) {
-            const u32 b = std::min(ends[k], bend);
-            FillDesc f;
-            f.base = ch.base + (u64)s0 * 30ull * e->cfg.seg_bytes;
-            f.seg0 = ch.seg0 + s0;
-            f.nseg = ns;
-            f.i0 = a;
-            f.i1 = b;
-            P.fills.push_back(f);
-            a = b;
-          }
+          if (g_nseg + ns > G) flush();
+          FillRun fr;
+          fr.base = ch.base + (u64)s0 * 30ull * e->cfg.seg_bytes;
+          fr.seg0 = ch.seg0 + s0;
+          fr.nseg = ns;
+          fr.lseg0 = g_nseg;
+          fr.pad = 0;
+          P.fruns.push_back(fr);
+          g_nseg += ns;
+          g_nq = std::max(g_nq, ch.nq);
         }
-        i = gend >= ch.nq ? ch.nq : gend;
       }
+      flush();
+      i = gend;
     }
   }
 }
@@ -1150,14 +1229,14 @@ static int size_set(BufSet& B, const PassPlan& P) {
 }
 
 // Bucket fill + sieve of pass P (descriptors at dsegs / druns) into buffer set B on stream st.
-static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, const RunDesc* druns,
+static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, const FillRun* druns,
                          const FillDesc* dfills, BufSet& B, hipStream_t st, DevAcc* acc, bool want_bitmap = false) {
   const u64 nseg = P.segs.size();
   const int cus = e->prop.multiProcessorCount;
   if (P.bcap) {
     hipLaunchKernelGGL(k_zero, dim3((u32)std::min<u64>((nseg + 255) / 256, 1024)), dim3(256), 0, st, B.bcnt, nseg);
     if (!P.fills.empty()) {
-      hipLaunchKernelGGL(k_bucket_fill, dim3((u32)P.fills.size()), dim3(FILL_BLOCK), 0, st, dfills, e->d_sp,
+      hipLaunchKernelGGL(k_bucket_fill, dim3((u32)P.fills.size()), dim3(FILL_BLOCK), 0, st, dfills, druns, e->d_sp,
                          e->d_spinv, B.bcnt, B.bdata, P.bcap, e->sbshift, e->d_flags);
       HIPCHK(hipGetLastError());
     }
@@ -1196,9 +1275,10 @@ static int enqueue_test(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, B
 static int upload_pass(wf_engine* e, const PassPlan& P) {
   int rc;
   if ((rc = grow(e->d_segs, e->seg_cap, P.segs.size())) != WF_OK) return rc;
-  if ((rc = grow(e->d_runs, e->run_cap, std::max<size_t>(P.runs.size(), 1))) != WF_OK) return rc;
+  if ((rc = grow(e->d_runs, e->run_cap, std::max<size_t>(P.fruns.size(), 1))) != WF_OK) return rc;
   HIPCHK(hipMemcpyAsync(e->d_segs, P.segs.data(), P.segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, e->st));
-  HIPCHK(hipMemcpyAsync(e->d_runs, P.runs.data(), P.runs.size() * sizeof(RunDesc), hipMemcpyHostToDevice, e->st));
+  if (!P.fruns.empty())
+    HIPCHK(hipMemcpyAsync(e->d_runs, P.fruns.data(), P.fruns.size() * sizeof(FillRun), hipMemcpyHostToDevice, e->st));
   if (!P.fills.empty()) {
     if ((rc = grow(e->d_fills, e->fill_cap, P.fills.size())) != WF_OK) return rc;
     HIPCHK(hipMemcpyAsync(e->d_fills, P.fills.data(), P.fills.size() * sizeof(FillDesc), hipMemcpyHostToDevice, e->st));
@@ -1264,7 +1344,7 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
     roff[k] = nruns;
     foff[k] = nfills;
     nsegs += plans[k].segs.size();
-    nruns += plans[k].runs.size();
+    nruns += plans[k].fruns.size();
     nfills += plans[k].fills.size();
   }
   e->last.plan_ms += ms_since(t0);
@@ -1280,8 +1360,8 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
     if ((rc = size_set(e->bs[k & 1], plans[k])) != WF_OK) return rc;
     HIPCHK(hipMemcpyAsync(e->d_segs + soff[k], plans[k].segs.data(), plans[k].segs.size() * sizeof(SegDesc),
                           hipMemcpyHostToDevice, e->st));
-    if (!plans[k].runs.empty())
-      HIPCHK(hipMemcpyAsync(e->d_runs + roff[k], plans[k].runs.data(), plans[k].runs.size() * sizeof(RunDesc),
+    if (!plans[k].fruns.empty())
+      HIPCHK(hipMemcpyAsync(e->d_runs + roff[k], plans[k].fruns.data(), plans[k].fruns.size() * sizeof(FillRun),
                             hipMemcpyHostToDevice, e->st));
   }
   HIPCHK(hipStreamSynchronize(e->st));
