@@ -486,6 +486,45 @@ __device__ __forceinline__ u64 mont64_dbl(u64 x, u64 n) {
   return (c || y >= n) ? y - n : y;
 }
 
+// A necessary condition for an odd composite n to be a base-2 strong probable prime.
+// Write n - 1 = d 2^s.  Either 2^d = 1 (mod n), and then ord_r(2) | d for every prime factor
+// r of n, or 2^(d 2^t) = -1 (mod n) for some t < s, and then ord_r(2) | d 2^(t+1) but not
+// d 2^t.  Either way ord_r(2) | n - 1 and all prime factors share the 2-adic valuation of
+// ord_r(2).  Checked for the factors r < 100 (products of them are reduced first, so each r
+// costs one 32-bit remainder); false means n is certainly not a strong probable prime.
+__device__ __forceinline__ bool psp_possible(u64 n) {
+  int v = -1;
+  bool ok = true;
+#define PSP_R(g, r, ord, v2)                                     \
+  if ((g) % (r##u) == 0u) {                                      \
+    if (n % (ord##u) != 1u || (v >= 0 && v != (v2))) ok = false; \
+    v = (v2);                                                    \
+  }
+  {
+    const u32 g = (u32)(n % 111546435u);   // 3 5 7 11 13 17 19 23
+    PSP_R(g, 3, 2, 1) PSP_R(g, 5, 4, 2) PSP_R(g, 7, 3, 0) PSP_R(g, 11, 10, 1)
+    PSP_R(g, 13, 12, 2) PSP_R(g, 17, 8, 3) PSP_R(g, 19, 18, 1) PSP_R(g, 23, 11, 0)
+  }
+  {
+    const u32 g = (u32)(n % 58642669u);    // 29 31 37 41 43
+    PSP_R(g, 29, 28, 2) PSP_R(g, 31, 5, 0) PSP_R(g, 37, 36, 2) PSP_R(g, 41, 20, 2) PSP_R(g, 43, 14, 1)
+  }
+  {
+    const u32 g = (u32)(n % 600662303u);   // 47 53 59 61 67
+    PSP_R(g, 47, 23, 0) PSP_R(g, 53, 52, 2) PSP_R(g, 59, 58, 1) PSP_R(g, 61, 60, 2) PSP_R(g, 67, 66, 1)
+  }
+  {
+    const u32 g = (u32)(n % 33984931u);    // 71 73 79 83
+    PSP_R(g, 71, 35, 0) PSP_R(g, 73, 9, 0) PSP_R(g, 79, 39, 0) PSP_R(g, 83, 82, 1)
+  }
+  {
+    const u32 g = (u32)(n % 8633u);        // 89 97
+    PSP_R(g, 89, 11, 0) PSP_R(g, 97, 48, 4)
+  }
+#undef PSP_R
+  return ok;
+}
+
 // true iff n is a base-2 strong probable prime (n odd, n >= 3)
 __device__ __forceinline__ bool sprp2_64(u64 n) {
   const u64 ninv = 0 - inv64(n);

@@ -666,39 +666,60 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
 }
 
 // Base-2 strong pseudoprimes: every odd composite n of the segments (wheel composites
-// from the sieve bitmap, and the odd multiples of 3 and 5) gets the exact SPRP-2 test;
-// the ones that pass are appended to out[] and counted per range.
-#define PSP_TILE 1024
-
+// from the sieve bitmap, and the odd multiples of 3 and 5) that passes the small-factor
+// condition psp_possible is queued in LDS; the workgroup then runs the exact SPRP-2 test
+// on the queue with every lane busy, and the n that pass are appended to out[] and counted
+// per range.
+#define PSP_TILE 4096
 
 __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u32 nseg,
                                              const u32* __restrict__ bitmap, u32 bmw, u32 tps,
                                              u64* __restrict__ out, u64 cap, unsigned long long* __restrict__ count,
                                              DevAcc* __restrict__ acc) {
+  __shared__ u64 queue[PSP_TILE];
+  __shared__ u32 qn;
+  const int lane = threadIdx.x & 63;
   const u64 ntiles = (u64)nseg * tps;
   for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const u32 sg = (u32)(t / tps);
     const u32 k = (u32)(t - (u64)sg * tps);
     const SegDesc d = segs[sg];
     const u32 nodd = 15u * d.nbytes;                 // odd offsets 1, 3, ..., 30 nbytes - 1
-    if (k * PSP_TILE >= nodd) continue;
-    for (u32 j = threadIdx.x; j < PSP_TILE; j += blockDim.x) {
-      const u32 i = k * PSP_TILE + j;
-      if (i >= nodd) break;
+    if (k * PSP_TILE >= nodd) continue;              // uniform across the workgroup
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
+    for (u32 j0 = 0; j0 < PSP_TILE; j0 += blockDim.x) {
+      const u32 i = k * PSP_TILE + j0 + threadIdx.x;
       const u32 off = 2u * i + 1u;
-      if (off < d.lo_off || off >= d.hi_off) continue;
       const u64 n = d.base + off;
-      bool comp;
-      const u32 r = off % 30u;
-      const int cls = c_W30[r];
-      if (cls < 0) comp = n > 5;                      // an odd multiple of 3 or 5
-      else comp = (bitmap[(u64)sg * bmw + off / 120u] >> (((off / 30u) & 3u) * 8u + (u32)cls)) & 1u;
-      if (comp && sprp2_64(n)) {
+      bool want = false;
+      if (i < nodd && off >= d.lo_off && off < d.hi_off) {
+        const int cls = c_W30[off % 30u];
+        const bool comp = cls < 0 ? n > 5
+                                  : ((bitmap[(u64)sg * bmw + off / 120u] >> (((off / 30u) & 3u) * 8u + (u32)cls)) & 1u);
+        want = comp && psp_possible(n);
+      }
+      // one LDS atomic per wavefront reserves the queue slots of its candidates
+      const u64 mask = __ballot(want);
+      if (mask) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        u32 base = 0;
+        if (lane == leader) base = atomicAdd(&qn, (u32)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (want) queue[base + __popcll(mask & ((1ull << lane) - 1))] = n;
+      }
+    }
+    __syncthreads();
+    const u32 m = qn;
+    for (u32 j = threadIdx.x; j < m; j += blockDim.x) {
+      const u64 n = queue[j];
+      if (sprp2_64(n)) {
         const unsigned long long kk = atomicAdd(count, 1ull);
         if (kk < cap) out[kk] = n;
         atomicAdd(&acc[d.rid].npsp, 1ull);
       }
     }
+    __syncthreads();   // the queue is refilled by the next tile
   }
 }
 
@@ -843,6 +864,9 @@ static void host_primes(u64 lim, std::vector<u32>& out) {
 }
 
 static int ensure_sieving_primes(wf_engine* e, u64 lim) {
+  // primes below 2^32 - 1 sieve every range below 2^64 (the table limit is capped there,
+  // so asking for more must not rebuild it)
+  if (lim > 0xFFFFFFFFull) lim = 0xFFFFFFFFull;
   if (lim <= e->sp_lim) return WF_OK;
   u64 want = std::max<u64>(lim, std::min<u64>(std::max<u64>(2 * e->sp_lim, 1u << 20), 0xFFFFFFFFull));
   if (want > 0xFFFFFFFFull) want = 0xFFFFFFFFull;

@@ -32,6 +32,8 @@ void emu_mont(int L, int full, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo
 
 int emu_sprp2(uint64_t n) { return sprp2_64(n) ? 1 : 0; }
 
+int emu_psp_possible(uint64_t n) { return psp_possible(n) ? 1 : 0; }
+
 void emu_m28(uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
   uint32_t x[4];
   m28_pow2(p, p - 1, ebits, x);
