@@ -123,12 +123,29 @@ def bench_c3(args):
     digest = 0
     for r in per:
         digest = (digest * 0x100000001B3 ^ r["cks"] ^ r["primes"]) & ((1 << 64) - 1)
+    compare = None
+    if args.c3_compare:
+        # the same first N ranges through the plain single-range path (one wf_scan each)
+        # and through one batched call: the gain of batching many uneven ranges
+        sub = ranges[:args.c3_compare]
+        t = time.perf_counter()
+        single = [eng.scan(lo, hi) for lo, hi in sub]
+        t_plain = time.perf_counter() - t
+        t = time.perf_counter()
+        bper, _ = eng.scan_batch(sub)
+        t_batch = time.perf_counter() - t
+        assert [(x["primes"], x["cks"]) for x in single] == [(x["primes"], x["cks"]) for x in bper]
+        nt = sum(x["tests"] for x in single)
+        compare = {"ranges": len(sub), "plain_primes_per_s": nt / t_plain, "batched_primes_per_s": nt / t_batch,
+                   "batched_over_plain": t_plain / t_batch}
+        print("c3 compare: %s" % compare, file=sys.stderr, flush=True)
     eng.close()
     return _engine_line("primes Wieferich-tested/sec, 4096 batched ranges", tests / (ms / 1e3), "primes tested/s",
                         ms, args, {"workload": "4096 ranges, starts U[1e12,1e18], lengths logU[1e6,1e9], seed "
                                    "0x243F6A8885A308D3", "engine": "batched", "ranges": len(ranges),
                                    "numbers": numbers, "tests": tests, "per_range_digest": "%016x" % digest,
                                    "autotuned": tuned, "w_products_per_s": prods / (ms / 1e3),
+                                   "plain_vs_batched_same_ranges": compare,
                                    "device_ms_per_step": sum(t["total_ms"] for t in tims) / len(tims),
                                    "host_plan_ms_per_step": sum(t["plan_ms"] for t in tims) / len(tims)})
 
@@ -233,6 +250,8 @@ def main():
     ap.add_argument("--queries", type=int, default=2000, help="c5: queries per rate")
     ap.add_argument("--rates", default="100,1000,5000", help="c5: Poisson arrival rates per second")
     ap.add_argument("--c4-span", type=float, default=2.0 ** 34, help="c4: numbers below 2^64 per step")
+    ap.add_argument("--c3-compare", type=int, default=0,
+                    help="c3: also time the first N ranges one wf_scan at a time against one batched call")
     ap.add_argument("--test-wgs", type=int, default=0,
                     help="batched: test workgroups per CU (0 = engine default)")
     args = ap.parse_args()

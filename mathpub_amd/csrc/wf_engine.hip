@@ -94,6 +94,7 @@ struct WheelLds {
   unsigned char m30[8 * 30 * 8];
   unsigned char qi[32];
   signed char w30[32];
+  unsigned char r30[8];
 };
 
 __device__ __forceinline__ void load_wheel(WheelLds& w) {
@@ -102,6 +103,7 @@ __device__ __forceinline__ void load_wheel(WheelLds& w) {
     w.qi[threadIdx.x] = c_QI[threadIdx.x];
     w.w30[threadIdx.x] = c_W30[threadIdx.x];
   }
+  if (threadIdx.x < 8) w.r30[threadIdx.x] = (unsigned char)c_R30[threadIdx.x];
 }
 
 // Byte index j (relative to the segment base) of the first mark of class c for a prime q
@@ -112,7 +114,7 @@ __device__ __forceinline__ void load_wheel(WheelLds& w) {
 __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c, const WheelLds& w) {
   const u32 o32 = (u32)o;
   const u32 m = w.m30[(qi * 30 + o32 % 30u) * 8 + c];
-  const u64 v = (u64)m * q + o32 - c_R30[c];
+  const u64 v = (u64)m * q + o32 - w.r30[c];
   if (v >= (30ull << 32)) return ~0ull;   // beyond any segment or run (q close to 2^32)
   return (u64)((u32)(v >> 1) * 0xEEEEEEEFu);
 }
@@ -149,6 +151,9 @@ __device__ __forceinline__ void walk_multiples(u64 o, u32 q, u64 span, const Whe
 #define FILL_STAGE 16384  // records a fill workgroup stages in LDS (64 KiB)
 #endif
 #define FILL_TARGET (FILL_STAGE / 2)  // expected records per fill work item (the host sizes items to it)
+#ifndef WALK_DIV
+#define WALK_DIV 8        // primes q > span / WALK_DIV walk their multiples instead of 8 class starts
+#endif
 #ifndef FILL_BLOCK
 #define FILL_BLOCK 512    // threads per fill workgroup
 #endif
@@ -241,7 +246,7 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
       const u64 span = 30ull * nbytes;
       u64 o;
       if (!first_offset(fr.base, q, minv, span, o)) continue;
-      if (!pairs && 8ull * q > span) {
+      if (!pairs && (u64)WALK_DIV * q > span) {
         walk_multiples(o, q, span, wl, [&](u32 j, int) { atomicAdd(&cur[fr.lseg0 + (j >> sbshift)], 1u); });
         continue;
       }
@@ -300,7 +305,7 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
           if (pos < bcap) bdata[(u64)gseg[sg] * bcap + pos] = rec;
         }
       };
-      if (!pairs && 8ull * q > span) {
+      if (!pairs && (u64)WALK_DIV * q > span) {
         walk_multiples(o, q, span, wl, put);
         continue;
       }
@@ -365,10 +370,18 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
     __syncthreads();
     const u32 nsmall = min(d.nq, q1idx);
     // tier A: one wavefront per prime, lanes stride over the marks of each class
-    for (u32 i = PRE_COUNT + wave; i < min(tierA, nsmall); i += nwaves) {
-      const u32 q = sp[i];
+    const u32 na = min(tierA, nsmall);
+    u32 qa = PRE_COUNT + wave < na ? sp[PRE_COUNT + wave] : 0;
+    u64 ma = PRE_COUNT + wave < na ? spinv[PRE_COUNT + wave] : 0;
+    for (u32 i = PRE_COUNT + wave; i < na; i += nwaves) {
+      const u32 q = qa;
+      const u64 minv = ma;
+      if (i + nwaves < na) {      // the wavefront's next prime, loaded while this one is sieved
+        qa = sp[i + nwaves];
+        ma = spinv[i + nwaves];
+      }
       u64 o;
-      if (!first_offset(d.base, q, spinv[i], span, o)) continue;
+      if (!first_offset(d.base, q, minv, span, o)) continue;
       const int qi = wl.qi[q % 30];
       const u32 jl = (u32)class_start(o, q, qi, lane & 7, wl);     // lane c (< 8) holds class c's start
       for (int c = 0; c < 8; c++) {
@@ -392,7 +405,7 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
         }
         u64 o;
         if (!first_offset(d.base, q, minv, span, o)) continue;
-        if (8ull * q > span) {
+        if ((u64)WALK_DIV * q > span) {
           walk_multiples(o, q, span, wl, [&](u32 j, int c) { atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c)); });
           continue;
         }
@@ -456,7 +469,7 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
       while (v) {
         const int b = __ffs(v) - 1;
         v &= v - 1;
-        if (k < pcap) o[k] = 120u * w + 30u * (b >> 3) + c_R30[b & 7];
+        if (k < pcap) o[k] = 120u * w + 30u * (b >> 3) + wl.r30[b & 7];
         k++;
       }
       pos += __shfl(incl, 63, 64);
@@ -678,6 +691,8 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
                                              DevAcc* __restrict__ acc) {
   __shared__ u64 queue[PSP_TILE];
   __shared__ u32 qn;
+  __shared__ signed char w30[32];
+  if (threadIdx.x < 30) w30[threadIdx.x] = c_W30[threadIdx.x];   // ordered by the first barrier below
   const int lane = threadIdx.x & 63;
   const u64 ntiles = (u64)nseg * tps;
   for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -694,7 +709,7 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
       const u64 n = d.base + off;
       bool want = false;
       if (i < nodd && off >= d.lo_off && off < d.hi_off) {
-        const int cls = c_W30[off % 30u];
+        const int cls = w30[off % 30u];
         const bool comp = cls < 0 ? n > 5
                                   : ((bitmap[(u64)sg * bmw + off / 120u] >> (((off / 30u) & 3u) * 8u + (u32)cls)) & 1u);
         want = comp && psp_possible(n);
