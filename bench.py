@@ -328,6 +328,12 @@ def main():
             roof = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_test<A_F64>",
                     "valu_fp64_issue_frac": tests * instr / (test_ms / 1e3) / FP64_PEAK_INSTR}
+        tj = os.path.join(ROOT, "profiles", "c2_k_test_traffic.json")
+        if roof["kernel"] == "k_test<A_F64>" and span == SPAN and os.path.exists(tj):
+            # HBM bytes per launch from the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+            # this workload, scaled by the same counters' reading of a plain 1 GiB copy
+            roof["traffic"] = json.load(open(tj))["hbm_bytes_per_launch"]
+            roof["traffic_unit"] = "bytes per launch (rocprofv3 --pmc, profiles/c2_hbm_traffic_pmc.md)"
         roof["avg_launch_ms"] = test_ms / launches
         roof["launches_per_step"] = launches
         roof["w_products_per_s"] = tests * wf_products(pbits) / (test_ms / 1e3)

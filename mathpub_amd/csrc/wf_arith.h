@@ -145,9 +145,13 @@ __device__ __forceinline__ void fp64_pow2x2(u64 p0, u64 e0, u64 p1, u64 e1, int 
   m1.ip = 1.0 / m1.p;
   const int nw = (ebits + W - 1) / W;
   int sh = (nw - 1) * W;
+  // the leading digit: x = 2^d directly when that is already below p
+  const int d0 = (int)(e0 >> sh), d1 = (int)(e1 >> sh);
   a0 = 1.0; b0 = 0.0; a1 = 1.0; b1 = 0.0;
-  f64_mul_pow2(a0, b0, (int)(e0 >> sh), m0);
-  f64_mul_pow2(a1, b1, (int)(e1 >> sh), m1);
+  if ((1ull << d0) < p0) a0 = __builtin_ldexp(1.0, d0);
+  else f64_mul_pow2(a0, b0, d0, m0);
+  if ((1ull << d1) < p1) a1 = __builtin_ldexp(1.0, d1);
+  else f64_mul_pow2(a1, b1, d1, m1);
   for (sh -= W; sh >= 0; sh -= W) {
 #pragma unroll
     for (int s = 0; s < W; s++) {

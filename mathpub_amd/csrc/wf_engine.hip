@@ -151,6 +151,13 @@ __device__ __forceinline__ void walk_multiples(u64 o, u32 q, u64 span, const Whe
 #define FILL_STAGE 16384  // records a fill workgroup stages in LDS (64 KiB)
 #endif
 #define FILL_TARGET (FILL_STAGE / 2)  // expected records per fill work item (the host sizes items to it)
+// Wave priority of the sieve-stage kernels.  In the pipelined path they share the CUs
+// with the previous pass's test kernel, which alone keeps the VALU nearly saturated; a
+// higher priority lets their (latency-bound) waves issue first, and the tests take the
+// slots they leave.  Without a co-running kernel it changes nothing.
+#ifndef SIEVE_PRIO
+#define SIEVE_PRIO 3
+#endif
 #ifndef WALK_DIV
 #define WALK_DIV 8        // primes q > span / WALK_DIV walk their multiples instead of 8 class starts
 #endif
@@ -222,6 +229,7 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
   __shared__ u32 wtot[FILL_BLOCK / 64];
   __shared__ u32 total;
   __shared__ WheelLds wl;
+  if (SIEVE_PRIO) __builtin_amdgcn_s_setprio(SIEVE_PRIO);
   load_wheel(wl);
   const FillDesc it = items[blockIdx.x];
   const u32 smask = (1u << sbshift) - 1;
@@ -351,6 +359,7 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
   extern __shared__ u32 seg[];
   __shared__ u32 wtot[16];
   __shared__ WheelLds wl;
+  if (SIEVE_PRIO) __builtin_amdgcn_s_setprio(SIEVE_PRIO);
   load_wheel(wl);   // the first barrier below orders it before every lookup
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -633,13 +642,16 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
       cur = d.rid;
     }
     const u32* po = off + (u64)sg * pcap;
+    // exponent length of the segment's largest number: no prime of the tile has a longer
+    // one, so the ladders of a wavefront agree without a cross-lane reduction
+    const u64 etop = d.base + d.hi_off - 2;
+    const int eb = etop ? 64 - __builtin_clzll(etop) : 1;
     if (ARITH == A_F64) {
       // two primes per lane: i and i + blockDim.x
       for (u32 base = beg; base < end; base += 2 * blockDim.x) {
         const u32 i0 = base + threadIdx.x, i1 = i0 + blockDim.x;
         const bool act0 = i0 < end, act1 = i1 < end;
         const u64 p0 = act0 ? d.base + po[i0] : idle, p1 = act1 ? d.base + po[i1] : idle;
-        const int eb = wave_max(max(64 - __builtin_clzll(p0 - 1), 64 - __builtin_clzll(p1 - 1)));
         double a0, b0, a1, b1;
         fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, eb, a0, b0, a1, b1);
         if (act0) account_padic(o, d.rid, p0, a0, b0, s);
@@ -650,7 +662,6 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
         const u32 i = base + threadIdx.x;
         const bool active = i < end;
         const u64 p = active ? d.base + po[i] : idle;
-        const int eb = wave_max(64 - __builtin_clzll(p - 1));
         u64 rhi, rlo;
         residue<ARITH>(p, eb, rhi, rlo);
         if (active) account(o, d.rid, p, rhi, rlo, s);

@@ -17,6 +17,16 @@ void emu_fp64(uint64_t p, uint64_t* hi, uint64_t* lo) {
   *lo = (uint64_t)r;
 }
 
+// two ladders at once (the kernel's form), with a wavefront-wide exponent length ebits
+// that may exceed either prime's own
+void emu_fp64x2(uint64_t p0, uint64_t p1, int ebits, uint64_t* r) {
+  double a0, b0, a1, b1;
+  fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, ebits, a0, b0, a1, b1);
+  const unsigned __int128 r0 = (unsigned __int128)(uint64_t)b0 * p0 + (uint64_t)a0;
+  const unsigned __int128 r1 = (unsigned __int128)(uint64_t)b1 * p1 + (uint64_t)a1;
+  r[0] = (uint64_t)(r0 >> 64); r[1] = (uint64_t)r0; r[2] = (uint64_t)(r1 >> 64); r[3] = (uint64_t)r1;
+}
+
 // the same through the L-limb Montgomery ladder; full = carry-keeping variant
 void emu_mont(int L, int full, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
   const uint64_t e = p - 1;

@@ -26,6 +26,7 @@ def emu():
     L.emu_sprp2.argtypes = [ctypes.c_uint64]
     L.emu_psp_possible.argtypes = [ctypes.c_uint64]
     L.emu_m28.argtypes = [ctypes.c_uint64, ctypes.c_int, P, P]
+    L.emu_fp64x2.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
     return L
 
 
@@ -52,6 +53,26 @@ def test_fp64_padic(emu):
         if p < 3:
             continue
         assert call(emu.emu_fp64, p) == pow(2, p - 1, p * p), p
+
+
+def test_fp64_two_ladders(emu):
+    """The kernel's two-ladder form: pairs of primes of different sizes, with the
+    wavefront's exponent length (the largest of the pair, or more), small p whose leading
+    window digit 2^d exceeds p, and p at the top of the class."""
+    ps = [3, 5, 7, 11, 13, 31, 37, 1093, 3511, (1 << 50) - 27, (1 << 49) + 1] + samples(2, 50, 6000, 7)
+    rng = random.Random(11)
+    out = (ctypes.c_uint64 * 4)()
+    for p0 in ps:
+        if p0 < 3:
+            continue
+        p1 = rng.choice(ps)
+        if p1 < 3:
+            p1 = 3
+        eb = max((p0 - 1).bit_length(), (p1 - 1).bit_length())
+        for e in (eb, min(50, eb + rng.randint(0, 7))):
+            emu.emu_fp64x2(p0, p1, e, out)
+            assert (out[0] << 64 | out[1]) == pow(2, p0 - 1, p0 * p0), (p0, p1, e)
+            assert (out[2] << 64 | out[3]) == pow(2, p1 - 1, p1 * p1), (p0, p1, e)
 
 
 @pytest.mark.parametrize("L,full,lo,hi", [(2, 0, 2, 31), (3, 0, 32, 47), (4, 0, 48, 63), (4, 1, 64, 64),
