@@ -3,8 +3,9 @@
  *
  * The engine runs the search loop of A001220/wieferich.c (jeffsponaugle/Mathpub) on one
  * AMD MI355X: a wheel-30 segmented sieve over [lo, hi) on the GPU, then for every odd
- * prime p the base-2 Fermat quotient q = (2^(p-1) mod p^2 - 1) / p, its checksum term
- * mix64(q ^ p * 0x9E3779B97F4A7C15), the FLT check q < p and the FOUND / NEAR events.
+ * prime p and every base b of the run the Fermat quotient q = (b^(p-1) mod p^2 - 1) / p,
+ * its checksum term mix64(q ^ p * 0x9E3779B97F4A7C15), the FLT check q < p and the
+ * FOUND / NEAR events.
  * Results are bit-identical to the reference CPU tool.
  *
  * The reference has no plugin API; the entry points below replace its two internal seams
@@ -14,7 +15,7 @@
  *                  sieve_chunk/enqueue_chunk/fold_chunk, A001220/cuda/wieferich_cuda.cu:1090-1271
  *   wf_scan_batch  (no counterpart: many uneven ranges in one launch)
  *   wf_quotients   fermat_quotient()     A001220/wieferich.c:939-949, GPU k_quot
- *                  A001220/cuda/wieferich_cuda.cu:976-983 (check / selftest)
+ *   (_base)        A001220/cuda/wieferich_cuda.cu:976-983 (check / selftest)
  *   wf_primes      primesieve_jump_to / primesieve_next_prime, A001220/wieferich.c:1138-1141
  *                  (the prime set of [lo, hi), for tests and the sieve benchmark)
  *   wf_serve_*     (no counterpart: a device-resident request loop)
@@ -90,7 +91,12 @@ const char *wf_strerror(int rc);
 /* Device name, CU count and the arithmetic path, for status lines. */
 int  wf_info(wf_engine *e, char *buf, int len);
 
-/* Bases of the run, ascending and distinct.  The GPU kernels implement base 2. */
+/* Bases of the run, ascending and distinct, 2 <= b < 2^32, at most WF_MAXB (the -b list of
+ * parse_bases, wieferich.c:816-849).  Results carry one checksum and one sum of q per base
+ * (index j = position in this list); pairs with p | b are skipped, as process_batch does
+ * (wieferich.c:1097).  Base 2 runs its own ladders; any other base uses windows of w bits
+ * with multipliers b^d < 2^32 (wieferich.c:99-107).  wf_config.arith = 1 supports base 2
+ * only (WF_E_UNSUPPORTED otherwise).  The default is the single base 2. */
 int  wf_set_bases(wf_engine *e, const uint32_t *bases, int nb);
 
 /* Scan one range: totals into *out, events sorted by (p, j) into ev[0..ev_cap).
@@ -109,6 +115,10 @@ int  wf_scan_batch(wf_engine *e, const wf_range *r, uint32_t n, int64_t near_T,
  * q = (r - 1) p^-1 mod 2^64, as fermat_quotient() (wieferich.c:939-949). */
 int  wf_quotients(wf_engine *e, const uint64_t *p, uint64_t *q, uint64_t *r_hi,
                   uint64_t *r_lo, uint64_t n);
+/* The same for any base 2 <= b < 2^32 (the check command, wieferich.c:1596-1650; r = 0
+ * when p | b). */
+int  wf_quotients_base(wf_engine *e, uint32_t b, const uint64_t *p, uint64_t *q,
+                       uint64_t *r_hi, uint64_t *r_lo, uint64_t n);
 
 /* The primes of [lo, hi) in ascending order into out[0..cap); *count = how many exist. */
 int  wf_primes(wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *out, uint64_t cap,

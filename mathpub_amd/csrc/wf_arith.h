@@ -165,6 +165,54 @@ __device__ __forceinline__ void fp64_pow2x2(u64 p0, u64 e0, u64 p1, u64 e1, int 
   f64_normalise(a1, b1, m1);
 }
 
+// x *= c for an integer 0 < c < 2^32 (any base's window multiplier b^d): a c and b c are
+// split exactly into two doubles (h + l, FMA), each reduced by one rounded quotient, and the
+// b digit takes the carry k1 and one more reduction.  Afterwards |a|, |b| <= p/2 (+ 2^-21 p).
+__device__ __forceinline__ void f64_mul_small(double& a, double& b, double c, const F64Mod& m) {
+  const double h1 = a * c;
+  const double l1 = __builtin_fma(a, c, -h1);
+  const double k1 = round_mul(h1, m.ip);                    // |k1| < 2^32
+  const double a1 = __builtin_fma(-k1, m.p, h1) + l1;       // a c - k1 p
+  const double h2 = b * c;
+  const double l2 = __builtin_fma(b, c, -h2);
+  const double k2 = round_mul(h2, m.ip);
+  const double b1 = (__builtin_fma(-k2, m.p, h2) + l2) + k1;   // b c - k2 p + k1
+  const double k3 = round_mul(b1, m.ip);
+  b = __builtin_fma(-k3, m.p, b1);
+  a = a1;
+}
+
+// Two ladders for a base b with window W: pw[d] = b^d < 2^32 for d < 2^W.  x = b^(leading
+// digit), then per window W squarings and one multiplication by b^d.
+template <int W>
+__device__ __forceinline__ void fp64_powc_x2(u64 p0, u64 e0, u64 p1, u64 e1, int ebits, const u32* pw,
+                                             double& a0, double& b0, double& a1, double& b1) {
+  F64Mod m0, m1;
+  m0.p = (double)p0;
+  m0.ip = 1.0 / m0.p;
+  m1.p = (double)p1;
+  m1.ip = 1.0 / m1.p;
+  const int nw = (ebits + W - 1) / W;
+  int sh = (nw - 1) * W;
+  const u32 c0 = pw[(int)(e0 >> sh)], c1 = pw[(int)(e1 >> sh)];
+  a0 = 1.0; b0 = 0.0; a1 = 1.0; b1 = 0.0;
+  if ((u64)c0 < p0) a0 = (double)c0;
+  else f64_mul_small(a0, b0, (double)c0, m0);
+  if ((u64)c1 < p1) a1 = (double)c1;
+  else f64_mul_small(a1, b1, (double)c1, m1);
+  for (sh -= W; sh >= 0; sh -= W) {
+#pragma unroll
+    for (int s = 0; s < W; s++) {
+      f64_sqr(a0, b0, m0);
+      f64_sqr(a1, b1, m1);
+    }
+    f64_mul_small(a0, b0, (double)pw[(int)((e0 >> sh) & ((1u << W) - 1))], m0);
+    f64_mul_small(a1, b1, (double)pw[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
+  }
+  f64_normalise(a0, b0, m0);
+  f64_normalise(a1, b1, m1);
+}
+
 // ---------------------------------------------------------------------------------
 // 32-bit limb Montgomery path
 // ---------------------------------------------------------------------------------
@@ -416,19 +464,25 @@ __device__ __forceinline__ void m28_out(const u32 (&x)[5], const u32 (&n)[5], u3
 }
 #undef M28_ROW
 
-// r = 2^e mod N = p^2 as four 32-bit words, for odd p >= 2^47.  ebits = the wave's longest
-// exponent length (shorter exponents skip their leading positions).
-__device__ __forceinline__ void m28_pow2(u64 p, u64 e, int ebits, u32 (&r)[4]) {
+// N = p^2 as five 28-bit limbs, -N^-1 mod 2^28, R mod N (R = 2^140) and 1/N in FP64.
+struct M28Mod {
+  u32 n[5];
+  u32 ninv;
+  unsigned __int128 rmodn;
+  double invN;
+};
+
+__device__ __forceinline__ void m28_setup(u64 p, M28Mod& m) {
   typedef unsigned __int128 u128;
   const u128 N = (u128)p * p;
-  u32 n[5];
-  for (int i = 0; i < 5; i++) n[i] = (u32)(N >> (28 * i)) & M28;
+  for (int i = 0; i < 5; i++) m.n[i] = (u32)(N >> (28 * i)) & M28;
   const u64 pinv = inv64(p);
-  const u32 ninv = (0u - (u32)pinv * (u32)pinv) & M28;
+  m.ninv = (0u - (u32)pinv * (u32)pinv) & M28;
+  m.invN = 1.0 / ((double)p * (double)p);
   // R mod N = 2^140 - Q N with Q = floor(2^140 / N) < 2^46 estimated in FP64 (relative
   // error < 2^-50, so off by at most one); the 192-bit difference is then brought into
   // [0, N) by at most two additions or subtractions of N.
-  const u64 Q = (u64)(1393796574908163946345982392040522594123776.0 / ((double)p * (double)p));
+  const u64 Q = (u64)(0x1p140 * m.invN);
   const u128 A = (u128)Q * (u64)N;                   // Q * N_lo
   const u128 B = (u128)Q * (u64)(N >> 64);           // Q * N_hi
   const u128 low = A + (B << 64);                    // Q N mod 2^128
@@ -444,20 +498,78 @@ __device__ __forceinline__ void m28_pow2(u64 p, u64 e, int ebits, u32 (&r)[4]) {
     vtop -= vlow < N ? 1 : 0;
     vlow -= N;
   }
+  m.rmodn = vlow;
+}
+
+// r = 2^e mod N = p^2 as four 32-bit words, for odd p >= 2^47.  ebits = the wave's longest
+// exponent length (shorter exponents skip their leading positions).
+__device__ __forceinline__ void m28_pow2(u64 p, u64 e, int ebits, u32 (&r)[4]) {
+  typedef unsigned __int128 u128;
+  M28Mod m;
+  m28_setup(p, m);
   // Montgomery form of 2: 2 (R mod N) < 2N
-  const u128 two = vlow << 1;                         // < 2^129: bit 128 is lost only if N >= 2^127
+  const u128 two = m.rmodn << 1;                      // < 2^129: bit 128 is lost only if N >= 2^127
   u32 x[5];
   for (int i = 0; i < 4; i++) x[i] = (u32)(two >> (28 * i)) & M28;
-  x[4] = (u32)(two >> 112) | ((vlow >> 127) ? (1u << 16) : 0u);
+  x[4] = (u32)(two >> 112) | ((m.rmodn >> 127) ? (1u << 16) : 0u);
   // ladder over the bits below this lane's top bit
   const int tb = 63 - __builtin_clzll(e);
   for (int i = ebits - 2; i >= 0; i--) {
     if (i >= tb) continue;
-    m28_sqr(x, n, ninv);
+    m28_sqr(x, m.n, m.ninv);
     const u32 bit = (u32)(e >> i) & 1u;
     x[0] <<= bit; x[1] <<= bit; x[2] <<= bit; x[3] <<= bit; x[4] <<= bit;
   }
-  m28_out(x, n, ninv, r);
+  m28_out(x, m.n, m.ninv, r);
+}
+
+// x = x c mod N for an integer c < 2^32 (x < 64 N on entry, < 3 N on exit).  The
+// Montgomery form of a product with a plain integer is that form times the integer, so no
+// REDC is needed: T = x c < 2^38 N, its quotient by N is estimated in FP64 from the top
+// three limbs (the lower ones move T / N by < 2^-16; the estimate is off by at most one)
+// and one less than the estimate is subtracted, column by column with signed carries.
+__device__ __forceinline__ void m28_mul_small(u32 (&x)[5], u32 c, const M28Mod& m) {
+  u64 t0 = (u64)x[0] * c, t1 = (u64)x[1] * c, t2 = (u64)x[2] * c, t3 = (u64)x[3] * c, t4 = (u64)x[4] * c;
+  t1 += t0 >> 28;
+  t2 += t1 >> 28;
+  t3 += t2 >> 28;
+  t4 += t3 >> 28;
+  const u64 l5 = t4 >> 28;                               // < 2^39
+  const u32 l3 = (u32)t3 & M28, l4 = (u32)t4 & M28;
+  const double td = ((double)l5 * 0x1p140 + (double)l4 * 0x1p112) + (double)l3 * 0x1p84;
+  double qd = __builtin_floor(td * m.invN);
+  qd = qd >= 1.0 ? qd - 1.0 : 0.0;
+  const u64 q = (u64)qd;                                  // < 2^38
+  const u64 q0 = q & M28, q1 = q >> 28;
+  const long long c0 = (long long)((u32)t0 & M28) - (long long)(q0 * m.n[0]);
+  const long long c1 = (long long)((u32)t1 & M28) - (long long)(q0 * m.n[1] + q1 * m.n[0]) + (c0 >> 28);
+  const long long c2 = (long long)((u32)t2 & M28) - (long long)(q0 * m.n[2] + q1 * m.n[1]) + (c1 >> 28);
+  const long long c3 = (long long)l3 - (long long)(q0 * m.n[3] + q1 * m.n[2]) + (c2 >> 28);
+  const long long c4 = (long long)l4 - (long long)(q0 * m.n[4] + q1 * m.n[3]) + (c3 >> 28);
+  // the result is below 3N < 2^130: limb 4 takes bits 112..139 and nothing is left above
+  x[0] = (u32)c0 & M28;
+  x[1] = (u32)c1 & M28;
+  x[2] = (u32)c2 & M28;
+  x[3] = (u32)c3 & M28;
+  x[4] = (u32)c4 & M28;
+}
+
+// r = b^e mod N = p^2 (four 32-bit words) for a base with window W: pw[d] = b^d < 2^32.
+template <int W>
+__device__ __forceinline__ void m28_powc(u64 p, u64 e, int ebits, const u32* pw, u32 (&r)[4]) {
+  M28Mod m;
+  m28_setup(p, m);
+  u32 x[5];                                          // Montgomery form of 1: R mod N < N
+  for (int i = 0; i < 5; i++) x[i] = (u32)(m.rmodn >> (28 * i)) & M28;
+  const int nw = (ebits + W - 1) / W;
+  int sh = (nw - 1) * W;
+  m28_mul_small(x, pw[(int)(e >> sh)], m);
+  for (sh -= W; sh >= 0; sh -= W) {
+#pragma unroll
+    for (int s = 0; s < W; s++) m28_sqr(x, m.n, m.ninv);
+    m28_mul_small(x, pw[(int)((e >> sh) & ((1u << W) - 1))], m);
+  }
+  m28_out(x, m.n, m.ninv, r);
 }
 
 }  // namespace wf

@@ -190,11 +190,19 @@ struct FillRun { u64 base; u32 seg0, nseg, lseg0, pad; };
 // nseg segments in all (<= RUN_MAX)
 struct FillDesc { u32 fr0, nfr, i0, i1, nseg, pad; };
 
-struct DevAcc {     // per range; every field is a 64-bit sum
-  unsigned long long primes, tests, near, flt, cks, sumq, nev, npsp;
+struct DevAcc {     // per range; every field is a 64-bit sum (checksums are per range and base)
+  unsigned long long primes, tests, near, flt, nev, npsp;
 };
 
-struct DevEvent { u32 type, rid; u64 p, q, r_hi, r_lo; };
+struct DevEvent { u32 type, rid, j, pad; u64 p, q, r_hi, r_lo; };
+
+// The bases of the run: base j, its window w_j (the largest w <= 5 with b^(2^w - 1) < 2^32,
+// as wieferich.c chooses it) and its powers b^d, d < 2^w.  Slot WF_MAXB holds the base of a
+// wf_quotients_base call.
+#define BPOW 32
+__constant__ u32 c_base[WF_MAXB + 1];
+__constant__ u32 c_bwin[WF_MAXB + 1];
+__constant__ u32 c_bpow[(WF_MAXB + 1) * BPOW];
 
 // first multiple offset of q at or after max(base, q^2), relative to base; returns false
 // if it lies at or beyond 'span' numbers
@@ -503,19 +511,23 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
 enum Arith { A_F64 = 0, A_M2, A_M3, A_M4, A_FULL, A_M28, A_NCLS };
 
 struct TestOut {
-  DevAcc* acc;
+  DevAcc* acc;               // per range
+  unsigned long long* cks;   // per range and base: [rid * nb + j]
+  unsigned long long* sumq;
   DevEvent* ev;
   u64 ev_cap;
   unsigned long long* nev;   // total events (device counter)
   i64 near_T;
+  u32 nb;                    // bases of the run
 };
 
-__device__ __forceinline__ void push_event(const TestOut& o, u32 type, u32 rid, u64 p, u64 q, u64 rhi, u64 rlo) {
+__device__ __forceinline__ void push_event(const TestOut& o, u32 type, u32 rid, u32 j, u64 p, u64 q, u64 rhi,
+                                           u64 rlo) {
   const unsigned long long k = atomicAdd(o.nev, 1ull);
   atomicAdd(&o.acc[rid].nev, 1ull);
   if (k < o.ev_cap) {
     DevEvent e;
-    e.type = type; e.rid = rid; e.p = p; e.q = q; e.r_hi = rhi; e.r_lo = rlo;
+    e.type = type; e.rid = rid; e.j = j; e.pad = 0; e.p = p; e.q = q; e.r_hi = rhi; e.r_lo = rlo;
     o.ev[k] = e;
   }
 }
@@ -529,60 +541,61 @@ __device__ __forceinline__ void padic_to_r(u64 p, double a, double b, u64& rhi, 
   rhi = hi + (rlo < lo);
 }
 
-// Account one tested prime exactly as finish_lanes (wieferich.c:1051-1075) and add the
-// lane's terms to its running sums.  r = r_hi:r_lo, q = (r - 1) p^-1 mod 2^64.
+// Account one tested (prime, base) pair exactly as finish_lanes (wieferich.c:1051-1075)
+// and add the lane's terms to its running sums.  r = r_hi:r_lo, q = (r - 1) p^-1 mod 2^64.
 struct LaneSums { u64 tests, flt, near, cks, sumq; };
 
-__device__ __forceinline__ void account(const TestOut& o, u32 rid, u64 p, u64 rhi, u64 rlo, LaneSums& s) {
+__device__ __forceinline__ void account(const TestOut& o, u32 rid, u32 j, u64 p, u64 rhi, u64 rlo, LaneSums& s) {
   const u64 pinv = inv64(p);
   const u64 q = (rlo - 1) * pinv;
   s.tests++;
   if (q >= p) {
     s.flt++;
-    push_event(o, WF_EV_FLT, rid, p, q, rhi, rlo);
+    push_event(o, WF_EV_FLT, rid, j, p, q, rhi, rlo);
     return;
   }
   s.cks += d_mix64(q ^ (p * GOLD));
   s.sumq += q;
   if (q == 0) {
-    push_event(o, WF_EV_FOUND, rid, p, q, rhi, rlo);
+    push_event(o, WF_EV_FOUND, rid, j, p, q, rhi, rlo);
   } else if (o.near_T > 0) {
     const i64 A = q <= p / 2 ? (i64)q : -(i64)(p - q);
     if ((A < 0 ? -A : A) <= o.near_T) {
       s.near++;
-      push_event(o, WF_EV_NEAR, rid, p, q, rhi, rlo);
+      push_event(o, WF_EV_NEAR, rid, j, p, q, rhi, rlo);
     }
   }
 }
 
-// The same from the p-adic digits of r = a + b p: for prime p, a = 1 and q = b exactly,
-// so p^-1 and r are only formed for an event or a failed FLT check.
-__device__ __forceinline__ void account_padic(const TestOut& o, u32 rid, u64 p, double a, double b, LaneSums& s) {
+// The same from the p-adic digits of r = a + b p: for prime p (not dividing the base), a = 1
+// and q = b exactly, so p^-1 and r are only formed for an event or a failed FLT check.
+__device__ __forceinline__ void account_padic(const TestOut& o, u32 rid, u32 j, u64 p, double a, double b,
+                                              LaneSums& s) {
   if (a == 1.0) {
     const u64 q = (u64)b;                      // < p: the FLT check holds
     s.tests++;
     s.cks += d_mix64(q ^ (p * GOLD));
     s.sumq += q;
     if (q == 0) {
-      push_event(o, WF_EV_FOUND, rid, p, 0, 0, 1);
+      push_event(o, WF_EV_FOUND, rid, j, p, 0, 0, 1);
     } else if (o.near_T > 0) {
       const i64 A = q <= p / 2 ? (i64)q : -(i64)(p - q);
       if ((A < 0 ? -A : A) <= o.near_T) {
         s.near++;
         const u64 lo = q * p, hi = __umul64hi(q, p);
         const u64 rlo = lo + 1;
-        push_event(o, WF_EV_NEAR, rid, p, q, hi + (rlo < lo), rlo);
+        push_event(o, WF_EV_NEAR, rid, j, p, q, hi + (rlo < lo), rlo);
       }
     }
     return;
   }
   u64 rhi, rlo;
   padic_to_r(p, a, b, rhi, rlo);
-  account(o, rid, p, rhi, rlo, s);
+  account(o, rid, j, p, rhi, rlo, s);
 }
 
-// Wavefront sums of a range's lane totals -> one atomic per counter (lane 0).
-__device__ __forceinline__ void flush_sums(const TestOut& o, u32 rid, LaneSums& s) {
+// Wavefront sums of a (range, base)'s lane totals -> one atomic per counter (lane 0).
+__device__ __forceinline__ void flush_sums(const TestOut& o, u32 rid, u32 j, LaneSums& s) {
   const u64 t = wave_sum(s.tests), f = wave_sum(s.flt), n = wave_sum(s.near);
   const u64 c = wave_sum(s.cks), q = wave_sum(s.sumq);
   if ((threadIdx.x & 63) == 0 && t) {
@@ -590,29 +603,54 @@ __device__ __forceinline__ void flush_sums(const TestOut& o, u32 rid, LaneSums& 
     atomicAdd(&a->tests, t);
     if (f) atomicAdd(&a->flt, f);
     if (n) atomicAdd(&a->near, n);
-    atomicAdd(&a->cks, c);
-    atomicAdd(&a->sumq, q);
+    atomicAdd(&o.cks[(u64)rid * o.nb + j], c);
+    atomicAdd(&o.sumq[(u64)rid * o.nb + j], q);
   }
   s.tests = s.flt = s.near = s.cks = s.sumq = 0;
 }
 
-// One prime per lane over the segments' prime lists; workgroups take segments in turn
-// and keep their sums until the range changes (uniform across the workgroup).
+// true if p divides the base b (then b^(p-1) = 0 mod p and the pair is skipped, as
+// process_batch does, wieferich.c:1097)
+__device__ __forceinline__ bool divides_base(u64 p, u32 b) { return p <= b && b % (u32)p == 0; }
+
+// r = b^(p-1) mod p^2 for one odd p of the class: base 2 by its own ladders, any other base
+// by windows of w bits with the powers pw[d] = b^d.
 template <int ARITH>
-__device__ __forceinline__ void residue(u64 p, int eb, u64& rhi, u64& rlo) {
+__device__ __forceinline__ void residue(u64 p, int eb, u32 b, int w, const u32* pw, u64& rhi, u64& rlo) {
   const u64 e = p - 1;
-  if (ARITH == 0) {
-    double a, b;
-    fp64_pow2<5>(p, e, eb, a, b);
-    padic_to_r(p, a, b, rhi, rlo);
+  if (ARITH == A_F64) {
+    double a0, b0, a1, b1;
+    if (b == 2) {
+      fp64_pow2x2<5>(p, e, 3, 2, eb, a0, b0, a1, b1);
+    } else {
+      switch (w) {
+        case 5: fp64_powc_x2<5>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
+        case 4: fp64_powc_x2<4>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
+        case 3: fp64_powc_x2<3>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
+        case 2: fp64_powc_x2<2>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
+        default: fp64_powc_x2<1>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
+      }
+    }
+    padic_to_r(p, a0, b0, rhi, rlo);
   } else if (ARITH == A_M28) {
     u32 x[4];
-    m28_pow2(p, e, eb, x);
+    if (b == 2) {
+      m28_pow2(p, e, eb, x);
+    } else {
+      switch (w) {
+        case 5: m28_powc<5>(p, e, eb, pw, x); break;
+        case 4: m28_powc<4>(p, e, eb, pw, x); break;
+        case 3: m28_powc<3>(p, e, eb, pw, x); break;
+        case 2: m28_powc<2>(p, e, eb, pw, x); break;
+        default: m28_powc<1>(p, e, eb, pw, x); break;
+      }
+    }
     rlo = (u64)x[0] | ((u64)x[1] << 32);
     rhi = (u64)x[2] | ((u64)x[3] << 32);
   } else {
-    constexpr int L = ARITH == 1 ? 2 : (ARITH == 2 ? 3 : 4);
-    constexpr bool FULL = ARITH == 4;
+    // the 32-bit limb ladders implement base 2 (the host rejects other bases for them)
+    constexpr int L = ARITH == A_M2 ? 2 : (ARITH == A_M3 ? 3 : 4);
+    constexpr bool FULL = ARITH == A_FULL;
     u32 x[L];
     mont_pow2<L, FULL>(p, e, eb, x);
     rlo = (u64)x[0] | ((u64)x[1] << 32);
@@ -620,27 +658,39 @@ __device__ __forceinline__ void residue(u64 p, int eb, u64& rhi, u64& rlo) {
   }
 }
 
+// Work item = one tile of up to 1024 primes of one segment, for one base.  Workgroups take
+// items in turn and keep their sums until the (range, base) changes (uniform across the
+// workgroup).  The bases' power tables are copied to LDS once per workgroup.
 template <int ARITH>
 __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, u32 nseg,
                                               const u32* __restrict__ off, const u32* __restrict__ cnt, u32 pcap,
                                               u32 tps, TestOut o) {
+  __shared__ u32 pw_all[WF_MAXB * BPOW];
+  for (u32 k = threadIdx.x; k < o.nb * BPOW; k += blockDim.x) pw_all[k] = c_bpow[k];
+  __syncthreads();
   LaneSums s = {0, 0, 0, 0, 0};
-  u32 cur = 0xFFFFFFFFu;
+  u32 cur = 0xFFFFFFFFu, curj = 0;
   // a valid modulus of the class for idle lanes
-  const u64 idle = ARITH == 0 ? 3ull : ARITH == 1 ? 3ull : ARITH == 2 ? 2147483659ull
-                 : ARITH == 3 ? 140737488355333ull : 0xFFFFFFFFFFFFFFC5ull;   // M4 / FULL / M28
-  const u64 ntiles = (u64)nseg * tps;
-  for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const u32 sg = (u32)(t / tps);
-    const u32 beg = (u32)(t - (u64)sg * tps) * TEST_TILE;
+  const u64 idle = ARITH == A_F64 ? 3ull : ARITH == A_M2 ? 3ull : ARITH == A_M3 ? 2147483659ull
+                 : ARITH == A_M4 ? 140737488355333ull : 0xFFFFFFFFFFFFFFC5ull;   // FULL / M28
+  const u64 nitems = (u64)nseg * tps * o.nb;
+  for (u64 t = blockIdx.x; t < nitems; t += gridDim.x) {
+    const u32 j = (u32)(t % o.nb);
+    const u64 tile = t / o.nb;
+    const u32 sg = (u32)(tile / tps);
+    const u32 beg = (u32)(tile - (u64)sg * tps) * TEST_TILE;
     const u32 n = cnt[sg];
     if (beg >= n) continue;                       // uniform across the workgroup
     const u32 end = min(n, beg + TEST_TILE);
     const SegDesc d = segs[sg];
-    if (d.rid != cur) {
-      if (cur != 0xFFFFFFFFu) flush_sums(o, cur, s);
+    if (d.rid != cur || j != curj) {
+      if (cur != 0xFFFFFFFFu) flush_sums(o, cur, curj, s);
       cur = d.rid;
+      curj = j;
     }
+    const u32 b = c_base[j];
+    const int w = (int)c_bwin[j];
+    const u32* pw = pw_all + j * BPOW;
     const u32* po = off + (u64)sg * pcap;
     // exponent length of the segment's largest number: no prime of the tile has a longer
     // one, so the ladders of a wavefront agree without a cross-lane reduction
@@ -653,9 +703,19 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
         const bool act0 = i0 < end, act1 = i1 < end;
         const u64 p0 = act0 ? d.base + po[i0] : idle, p1 = act1 ? d.base + po[i1] : idle;
         double a0, b0, a1, b1;
-        fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, eb, a0, b0, a1, b1);
-        if (act0) account_padic(o, d.rid, p0, a0, b0, s);
-        if (act1) account_padic(o, d.rid, p1, a1, b1, s);
+        if (b == 2) {
+          fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, eb, a0, b0, a1, b1);
+        } else {
+          switch (w) {
+            case 5: fp64_powc_x2<5>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 4: fp64_powc_x2<4>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 3: fp64_powc_x2<3>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 2: fp64_powc_x2<2>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            default: fp64_powc_x2<1>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+          }
+        }
+        if (act0 && !divides_base(p0, b)) account_padic(o, d.rid, j, p0, a0, b0, s);
+        if (act1 && !divides_base(p1, b)) account_padic(o, d.rid, j, p1, a1, b1, s);
       }
     } else {
       for (u32 base = beg; base < end; base += blockDim.x) {
@@ -663,27 +723,31 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
         const bool active = i < end;
         const u64 p = active ? d.base + po[i] : idle;
         u64 rhi, rlo;
-        residue<ARITH>(p, eb, rhi, rlo);
-        if (active) account(o, d.rid, p, rhi, rlo, s);
+        residue<ARITH>(p, eb, b, w, pw, rhi, rlo);
+        if (active && !divides_base(p, b)) account(o, d.rid, j, p, rhi, rlo, s);
       }
     }
   }
-  if (cur != 0xFFFFFFFFu) flush_sums(o, cur, s);
+  if (cur != 0xFFFFFFFFu) flush_sums(o, cur, curj, s);
 }
 
-// Quotients of arbitrary odd p (check / selftest): per-lane choice of the path, the
-// scan's classes (int32 = 1 selects the 32-bit Montgomery ladders above 2^50).
+// Quotients of arbitrary odd p (check / selftest) for the base in slot `slot` of the base
+// tables: per-lane choice of the path, the scan's classes (int32 = 1 selects the 32-bit
+// Montgomery ladders above 2^50, base 2 only).
 __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u64* __restrict__ rhi_out,
-                       u64* __restrict__ rlo_out, u64 n, int int32) {
+                       u64* __restrict__ rlo_out, u64 n, int int32, u32 slot) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const u64 p = p_in[i];
   const int eb = 64 - __builtin_clzll(p - 1);
+  const u32 b = c_base[slot];
+  const int w = (int)c_bwin[slot];
+  const u32* pw = c_bpow + slot * BPOW;
   u64 rhi, rlo;
-  if (p < (1ull << 50)) residue<A_F64>(p, eb, rhi, rlo);
-  else if (!int32) residue<A_M28>(p, eb, rhi, rlo);
-  else if (p < (1ull << 63)) residue<A_M4>(p, eb, rhi, rlo);
-  else residue<A_FULL>(p, eb, rhi, rlo);
+  if (p < (1ull << 50)) residue<A_F64>(p, eb, b, w, pw, rhi, rlo);
+  else if (!int32) residue<A_M28>(p, eb, b, w, pw, rhi, rlo);
+  else if (p < (1ull << 63)) residue<A_M4>(p, eb, b, w, pw, rhi, rlo);
+  else residue<A_FULL>(p, eb, b, w, pw, rhi, rlo);
   q_out[i] = (rlo - 1) * inv64(p);
   rhi_out[i] = rhi;
   rlo_out[i] = rlo;
@@ -781,6 +845,9 @@ struct wf_engine {
   int sbshift = 15;
   int nb = 1;
   u32 bases[WF_MAXB] = {2};
+  unsigned long long* d_cks = nullptr;    // per range and base (checksums, sums of q)
+  unsigned long long* d_sumq = nullptr;
+  u64 cks_cap = 0, sumq_cap = 0;
   // sieving primes (odd primes >= 7, ascending) and prefix sums of 1/q
   std::vector<u32> h_sp;
   std::vector<double> h_inv_sum;
@@ -933,6 +1000,8 @@ static int grow(T*& ptr, u64& cap, u64 need) {
   return WF_OK;
 }
 
+static int put_base(u32 k, u32 b);
+
 extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
   if (!out) return WF_E_ARG;
   *out = nullptr;
@@ -991,7 +1060,7 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
       }
   }
   if (hipMemcpyToSymbol(HIP_SYMBOL(c_M30), m30, sizeof m30) != hipSuccess ||
-      hipMemcpyToSymbol(HIP_SYMBOL(c_QI), qi, sizeof qi) != hipSuccess) {
+      hipMemcpyToSymbol(HIP_SYMBOL(c_QI), qi, sizeof qi) != hipSuccess || put_base(0, 2) != WF_OK) {
     wf_close(e);
     return WF_E_DEVICE;
   }
@@ -1055,6 +1124,8 @@ extern "C" void wf_close(wf_engine* e) {
   (void)hipFree(e->d_pre);
   (void)hipFree(e->d_flags);
   (void)hipFree(e->d_acc);
+  (void)hipFree(e->d_cks);
+  (void)hipFree(e->d_sumq);
   (void)hipFree(e->d_ev);
   (void)hipFree(e->d_ctr);
   for (int i = 0; i < 4; i++)
@@ -1074,11 +1145,49 @@ extern "C" int wf_info(wf_engine* e, char* buf, int len) {
   return WF_OK;
 }
 
+// Window and powers of base b: the largest w <= 5 with b^(2^w - 1) < 2^32 (the rule of
+// wieferich.c's bases_setup, A001220/wieferich.c:99-107) and pw[d] = b^d for d < 2^w.
+static int base_window(u32 b, u32* pw) {
+  int w = 5;
+  while (w > 1) {
+    unsigned __int128 x = 1;
+    bool fits = true;
+    for (int k = 1; k < (1 << w) && fits; k++) {
+      x *= b;
+      fits = (x >> 32) == 0;
+    }
+    if (fits) break;
+    w--;
+  }
+  u64 x = 1;
+  for (int d = 0; d < BPOW; d++) {
+    pw[d] = d < (1 << w) ? (u32)x : 0;
+    x *= b;
+  }
+  return w;
+}
+
+// Upload base b's table into slot k of the constant tables.
+static int put_base(u32 k, u32 b) {
+  u32 pw[BPOW];
+  const u32 w = (u32)base_window(b, pw);
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_base), &b, sizeof b, k * sizeof(u32)) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_bwin), &w, sizeof w, k * sizeof(u32)) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_bpow), pw, sizeof pw, (size_t)k * BPOW * sizeof(u32)) != hipSuccess)
+    return WF_E_DEVICE;
+  return WF_OK;
+}
+
 extern "C" int wf_set_bases(wf_engine* e, const u32* bases, int nb) {
   if (!e || !bases || nb < 1 || nb > WF_MAXB) return WF_E_ARG;
   for (int i = 0; i < nb; i++)
     if (bases[i] < 2 || (i && bases[i] <= bases[i - 1])) return WF_E_ARG;
-  if (nb != 1 || bases[0] != 2) return WF_E_UNSUPPORTED;
+  if (e->srv) return WF_E_BUSY;
+  HIPCHK(hipSetDevice(e->dev));
+  for (int i = 0; i < nb; i++) {
+    const int rc = put_base((u32)i, bases[i]);
+    if (rc != WF_OK) return rc;
+  }
   e->nb = nb;
   memcpy(e->bases, bases, (size_t)nb * sizeof(u32));
   return WF_OK;
@@ -1472,6 +1581,7 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
 
 static bool cmp_dev_event(const DevEvent& a, const DevEvent& b) {
   if (a.p != b.p) return a.p < b.p;
+  if (a.j != b.j) return a.j < b.j;
   return a.type < b.type;
 }
 
@@ -1479,7 +1589,8 @@ static bool cmp_dev_event(const DevEvent& a, const DevEvent& b) {
 static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_result* res, wf_event* ev,
                        u64 ev_cap, bool batched, bool want_psp = false, u64* psp = nullptr, u64 psp_cap = 0) {
   if (!e || !r || !res || n == 0) return WF_E_ARG;
-  if (e->nb != 1 || e->bases[0] != 2) return WF_E_UNSUPPORTED;
+  // the 32-bit limb ladders implement base 2 only
+  if (e->cfg.arith == 1 && (e->nb != 1 || e->bases[0] != 2)) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
   u64 maxtop = 0;
   for (u32 i = 0; i < n; i++) {
@@ -1492,12 +1603,20 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   if ((rc = grow(e->d_acc, e->acc_cap, n)) != WF_OK) return rc;
   const u64 want_ev = std::max<u64>(ev_cap, 1 << 16);
   if ((rc = grow(e->d_ev, e->dev_ev_cap, want_ev)) != WF_OK) return rc;
+  const u64 ncks = (u64)n * e->nb;
+  if ((rc = grow(e->d_cks, e->cks_cap, ncks)) != WF_OK) return rc;
+  if ((rc = grow(e->d_sumq, e->sumq_cap, ncks)) != WF_OK) return rc;
   HIPCHK(hipMemsetAsync(e->d_acc, 0, n * sizeof(DevAcc), e->st));
+  HIPCHK(hipMemsetAsync(e->d_cks, 0, ncks * sizeof(unsigned long long), e->st));
+  HIPCHK(hipMemsetAsync(e->d_sumq, 0, ncks * sizeof(unsigned long long), e->st));
   HIPCHK(hipMemsetAsync(e->d_ctr, 0, 3 * sizeof(unsigned long long), e->st));
   if (want_psp && (rc = grow(e->d_psp, e->psp_cap, std::max<u64>(psp_cap, 1 << 16))) != WF_OK) return rc;
   HIPCHK(hipMemsetAsync(e->d_flags, 0, sizeof(u32), e->st));
   TestOut to;
   to.acc = e->d_acc;
+  to.cks = e->d_cks;
+  to.sumq = e->d_sumq;
+  to.nb = (u32)e->nb;
   to.ev = e->d_ev;
   to.ev_cap = e->dev_ev_cap;
   to.nev = e->d_ctr + 1;
@@ -1544,14 +1663,29 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   }
   // results
   std::vector<DevAcc> acc(n);
+  std::vector<unsigned long long> hcks(ncks), hsumq(ncks);
   unsigned long long nev = 0;
   HIPCHK(hipMemcpyAsync(acc.data(), e->d_acc, n * sizeof(DevAcc), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipMemcpyAsync(hcks.data(), e->d_cks, ncks * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipMemcpyAsync(hsumq.data(), e->d_sumq, ncks * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipMemcpyAsync(&nev, e->d_ctr + 1, sizeof nev, hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipStreamSynchronize(e->st));
   const u64 got = std::min<u64>(nev, e->dev_ev_cap);
   std::vector<DevEvent> dev(got);
   if (got) HIPCHK(hipMemcpy(dev.data(), e->d_ev, got * sizeof(DevEvent), hipMemcpyDeviceToHost));
+  // p = 2 is counted, never tested; it is a solution for every base b = 1 (mod 4), which
+  // scan_chunk records as a hit (wieferich.c:1143-1146)
+  for (u32 i = 0; i < n; i++)
+    if (r[i].lo <= 2 && (r[i].hi == 0 || r[i].hi > 2))
+      for (int j = 0; j < e->nb; j++)
+        if (e->bases[j] % 4 == 1) {
+          DevEvent x;
+          x.type = WF_EV_FOUND; x.rid = i; x.j = (u32)j; x.pad = 0; x.p = 2; x.q = 0; x.r_hi = 0; x.r_lo = 1;
+          dev.push_back(x);
+          acc[i].nev++;
+        }
   std::sort(dev.begin(), dev.end(), cmp_dev_event);
+  const u64 ndev = dev.size();
   u64 tested = 0;
   for (u32 i = 0; i < n; i++) {
     wf_result& o = res[i];
@@ -1562,14 +1696,16 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
     o.tests = acc[i].tests;
     o.near = acc[i].near;
     o.flt = acc[i].flt;
-    o.cks[0] = acc[i].cks;
-    o.sumq[0] = acc[i].sumq;
+    for (int j = 0; j < e->nb; j++) {
+      o.cks[j] = hcks[(u64)i * e->nb + j];
+      o.sumq[j] = hsumq[(u64)i * e->nb + j];
+    }
     o.nev = acc[i].nev;
     tested += o.tests;
   }
-  for (u64 k = 0; k < got && k < ev_cap; k++) {
+  for (u64 k = 0; k < ndev && k < ev_cap; k++) {
     ev[k].type = dev[k].type;
-    ev[k].j = 0;
+    ev[k].j = dev[k].j;
     ev[k].p = dev[k].p;
     ev[k].q = dev[k].q;
     ev[k].r_hi = dev[k].r_hi;
@@ -1604,19 +1740,22 @@ extern "C" int wf_scan_batch(wf_engine* e, const wf_range* r, u32 n, int64_t nea
   return scan_ranges(e, r, n, near_T, per_range, ev, ev_cap, true);
 }
 
-extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
-  if (!e || !p || !q || !r_hi || !r_lo) return WF_E_ARG;
+extern "C" int wf_quotients_base(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
+  if (!e || !p || !q || !r_hi || !r_lo || b < 2) return WF_E_ARG;
   if (n == 0) return WF_OK;
   for (u64 i = 0; i < n; i++)
     if (p[i] < 3 || !(p[i] & 1)) return WF_E_ARG;
+  if (e->cfg.arith == 1 && b != 2) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
+  int rc;
+  if ((rc = put_base(WF_MAXB, b)) != WF_OK) return rc;
   u64* d = nullptr;
   if (hipMalloc(&d, 4 * n * sizeof(u64)) != hipSuccess) return WF_E_NOMEM;
-  int rc = WF_OK;
+  rc = WF_OK;
   if (hipMemcpyAsync(d, p, n * sizeof(u64), hipMemcpyHostToDevice, e->st) != hipSuccess) rc = WF_E_DEVICE;
   if (rc == WF_OK) {
     hipLaunchKernelGGL(k_quot, dim3((u32)((n + 255) / 256)), dim3(256), 0, e->st, d, d + n, d + 2 * n, d + 3 * n, n,
-                       e->cfg.arith == 1 ? 1 : 0);
+                       e->cfg.arith == 1 ? 1 : 0, (u32)WF_MAXB);
     if (hipGetLastError() != hipSuccess) rc = WF_E_DEVICE;
   }
   if (rc == WF_OK && (hipMemcpyAsync(q, d + n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st) != hipSuccess ||
@@ -1626,6 +1765,10 @@ extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* 
     rc = WF_E_DEVICE;
   (void)hipFree(d);
   return rc;
+}
+
+extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
+  return wf_quotients_base(e, 2, p, q, r_hi, r_lo, n);
 }
 
 extern "C" int wf_primes(wf_engine* e, u64 lo, u64 hi, u64* out, u64 cap, u64* count) {
@@ -1774,6 +1917,9 @@ struct Server {
   std::unordered_map<u64, ServeReq> reqs;
   u64 next_ticket = 1;
   DevAcc* d_acc = nullptr;        // per slot
+  unsigned long long* d_cks = nullptr;    // per slot and base
+  unsigned long long* d_sumq = nullptr;
+  u32 nb = 1;
   u64 steps = 0;
 };
 
@@ -1806,6 +1952,9 @@ static void serve_loop(wf_engine* e) {
     rc = ensure_sieving_primes(e, std::max<u64>(isqrt_u64(maxtop) + 1, 64));
     TestOut to;
     to.acc = S->d_acc;
+    to.cks = S->d_cks;
+    to.sumq = S->d_sumq;
+    to.nb = S->nb;
     to.ev = e->d_ev;
     to.ev_cap = e->dev_ev_cap;
     to.nev = e->d_ctr + 1;
@@ -1817,7 +1966,12 @@ static void serve_loop(wf_engine* e) {
       if (!ps.empty()) rc = run_pieces(e, ps, cls, to);
     }
     std::vector<DevAcc> acc(S->slots);
-    if (rc == WF_OK && hipMemcpy(acc.data(), S->d_acc, S->slots * sizeof(DevAcc), hipMemcpyDeviceToHost) != hipSuccess)
+    std::vector<unsigned long long> cks((size_t)S->slots * S->nb), sumq((size_t)S->slots * S->nb);
+    if (rc == WF_OK &&
+        (hipMemcpy(acc.data(), S->d_acc, S->slots * sizeof(DevAcc), hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(cks.data(), S->d_cks, cks.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(sumq.data(), S->d_sumq, sumq.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) !=
+             hipSuccess))
       rc = WF_E_DEVICE;
     std::lock_guard<std::mutex> lk(S->mu);
     S->steps++;
@@ -1838,11 +1992,17 @@ static void serve_loop(wf_engine* e) {
       q.res.tests = a.tests;
       q.res.near = a.near;
       q.res.flt = a.flt;
-      q.res.cks[0] = a.cks;
-      q.res.sumq[0] = a.sumq;
+      for (u32 j = 0; j < S->nb; j++) {
+        q.res.cks[j] = cks[(size_t)q.slot * S->nb + j];
+        q.res.sumq[j] = sumq[(size_t)q.slot * S->nb + j];
+      }
       q.res.nev = a.nev;
+      if (q.lo <= 2 && (q.hi == 0 || q.hi > 2))      // the p = 2 solutions, as in wf_scan
+        for (int j = 0; j < e->nb; j++) q.res.nev += e->bases[j] % 4 == 1 ? 1 : 0;
       q.done = true;
       (void)hipMemset(S->d_acc + q.slot, 0, sizeof(DevAcc));
+      (void)hipMemset(S->d_cks + (size_t)q.slot * S->nb, 0, S->nb * sizeof(unsigned long long));
+      (void)hipMemset(S->d_sumq + (size_t)q.slot * S->nb, 0, S->nb * sizeof(unsigned long long));
       S->slot_ticket[q.slot] = 0;
       S->active--;
     }
@@ -1852,7 +2012,7 @@ static void serve_loop(wf_engine* e) {
 
 extern "C" int wf_serve_start(wf_engine* e, u32 slots, u64 hbm_budget) {
   if (!e || slots == 0 || slots > 65536 || e->srv) return WF_E_ARG;
-  if (e->nb != 1 || e->bases[0] != 2) return WF_E_UNSUPPORTED;
+  if (e->cfg.arith == 1 && (e->nb != 1 || e->bases[0] != 2)) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
   Server* S = new Server();
   S->slots = slots;
@@ -1866,11 +2026,23 @@ extern "C" int wf_serve_start(wf_engine* e, u32 slots, u64 hbm_budget) {
   S->step_numbers = (u64)std::min(2.0e10, (double)hbm_budget / per_number);
   S->min_piece = std::max<u64>(30ull * e->cfg.seg_bytes, std::min<u64>(S->step_numbers / 64, 200000000ull));
   if (S->step_numbers < S->min_piece) { delete S; return WF_E_ARG; }
+  S->nb = (u32)e->nb;
+  const size_t nck = (size_t)slots * S->nb * sizeof(unsigned long long);
   if (hipMalloc(&S->d_acc, slots * sizeof(DevAcc)) != hipSuccess) { delete S; return WF_E_NOMEM; }
+  if (hipMalloc(&S->d_cks, nck) != hipSuccess || hipMalloc(&S->d_sumq, nck) != hipSuccess ||
+      hipMemset(S->d_cks, 0, nck) != hipSuccess || hipMemset(S->d_sumq, 0, nck) != hipSuccess) {
+    (void)hipFree(S->d_acc);
+    (void)hipFree(S->d_cks);
+    (void)hipFree(S->d_sumq);
+    delete S;
+    return WF_E_NOMEM;
+  }
   int rc;
   if ((rc = grow(e->d_ev, e->dev_ev_cap, 1 << 16)) != WF_OK || hipMemset(S->d_acc, 0, slots * sizeof(DevAcc)) != hipSuccess ||
       hipMemset(e->d_flags, 0, sizeof(u32)) != hipSuccess || hipMemset(e->d_ctr, 0, 3 * sizeof(unsigned long long)) != hipSuccess) {
     (void)hipFree(S->d_acc);
+    (void)hipFree(S->d_cks);
+    (void)hipFree(S->d_sumq);
     delete S;
     return rc != WF_OK ? rc : WF_E_DEVICE;
   }
@@ -1929,6 +2101,8 @@ extern "C" int wf_serve_stop(wf_engine* e) {
   S->cv.notify_all();
   if (S->th.joinable()) S->th.join();
   (void)hipFree(S->d_acc);
+  (void)hipFree(S->d_cks);
+  (void)hipFree(S->d_sumq);
   const int err = S->err;
   delete S;
   e->srv = nullptr;

@@ -1,7 +1,7 @@
 /*
  * wf.c -- command-line front end of the MI355X Wieferich engine (libwf.so, include/wf.h).
  *
- * A drop-in for A001220/wieferich.c's command line on the base-2 path:
+ * A drop-in for A001220/wieferich.c's command line:
  *
  *   wf scan [START] END [-b BASES] [-c CHUNK] [-n T] [-S FILE] [-i SECS] [-L FILE] [-g DEV] [-q] [-Q]
  *   wf check P [-b BASES] [-g DEV]
@@ -13,10 +13,11 @@
  * (:1389-1394), the chunk log line "C lo hi primes b:cks" (:1166-1172), the checkpoint
  * file (:1223-1316, resumable by either tool), the FOUND / NEAR / FLT-ERROR / CONFIRMED
  * lines (:1009, :1060, :1071, :1350-1352), SUMMARY (:1505-1523), exit statuses 0 / 1 /
- * 2 / 3 (:271-280, :1535-1544, :1591).  Differences: the default base list is "2" (the
- * GPU kernels implement base 2), the default CHUNK is 10^10 as in the GPU tool
- * (wieferich_cuda.cu:1591), END may be 2^64 (the reference stops at 2^63), -Q prints
- * the sum of the quotients, and -t/-T are accepted and ignored (no host sieve threads).
+ * 2 / 3 (:271-280, :1535-1544, :1591), the default base lists ("oeis" for scan and bench,
+ * "known" for check) and the selftest's three parts (:1821-1900).  Differences: the
+ * default CHUNK is 10^10 as in the GPU tool (wieferich_cuda.cu:1591), END may be 2^64 (the
+ * reference stops at 2^63), -Q prints the sum of the quotients, and -t/-T are accepted
+ * and ignored (no host sieve threads).
  * Chunks are computed in order, so the frontier is the number of completed chunks.
  */
 #define _GNU_SOURCE
@@ -617,7 +618,8 @@ static void usage(void)
           "       wf check P [-b BASES] [-g DEV]\n"
           "       wf bench [N] [-d SPAN] [-b BASES] [-g DEV]\n"
           "       wf selftest [LIMIT] [-g DEV]\n"
-          "BASES: list/ranges like 3,5,7 or 2-30, 'oeis', 'known' (the GPU engine implements base 2; default 2)\n"
+          "BASES: list/ranges like 3,5,7 or 2-30; 'oeis' (default) = 3,5,6,7,10,12,13,14,15,17,18,19,20,22,23,26,30;\n"
+          "       'known' = the bases 2..30 that are not perfect powers (default of check)\n"
           "END may be 2^64\n", stderr);
     exit(2);
 }
@@ -627,7 +629,7 @@ static int cmd_scan(int argc, char **argv)
     u128 pos[2];
     int npos = 0, interval = 60;
     u64 chunk = 10000000000ULL;
-    const char *state = NULL, *logf = NULL, *bases = "2";
+    const char *state = NULL, *logf = NULL, *bases = "oeis";
     bool quiet = false, sumq = false;
     for (int i = 0; i < argc; i++) {
         const char *a = argv[i];
@@ -739,7 +741,7 @@ static u128 powmod_ref(u64 b, u64 e, u128 m)
 static int cmd_check(int argc, char **argv)
 {
     u64 p = 0;
-    const char *bases = "2";
+    const char *bases = "known";
     bool havep = false;
     for (int i = 0; i < argc; i++) {
         if (!strcmp(argv[i], "-b") && i + 1 < argc) bases = argv[++i];
@@ -758,8 +760,8 @@ static int cmd_check(int argc, char **argv)
         const u32 b = g_b[j];
         if (b % p == 0) { printf("base %u: p divides b\n", b); continue; }
         u64 q, rh, rl;
-        int rc = wf_quotients(g_eng, &p, &q, &rh, &rl, 1);
-        if (rc) die("wf_quotients: %s", wf_strerror(rc));
+        int rc = wf_quotients_base(g_eng, b, &p, &q, &rh, &rl, 1);
+        if (rc) die("wf_quotients_base: %s", wf_strerror(rc));
         const u128 rf = ((u128)rh << 64) | rl;
         const u128 rr = powmod_ref(b, p - 1, (u128)p * p);
         const u64 qr = (u64)((rr - 1) / p);
@@ -787,7 +789,7 @@ static double li_diff(double x, double y)
 static int cmd_bench(int argc, char **argv)
 {
     u64 n0 = 1000000000000000ULL, span = 10000000000ULL;
-    const char *bases = "2";
+    const char *bases = "oeis";
     bool haven = false;
     for (int i = 0; i < argc; i++) {
         if (!strcmp(argv[i], "-d") && i + 1 < argc) span = parse_num(argv[++i]);
@@ -835,6 +837,67 @@ static const struct { u64 lo, hi, primes, cks; } CKWIN2[] = {
     {9223372036852775808ULL, 9223372036854775808ULL, 45777ULL, 0x50454bd548214c98ULL},
 };
 
+/* The checksum windows of A001220/wieferich_cks.h:2-4 for the "oeis" bases (17 values per
+ * window, in OEIS_BASES order), the reference's own cross-check data. */
+static const struct { u64 lo, hi, primes; u64 cks[17]; } CKWIN_OEIS[] = {
+    {1000000000000000ULL, 1000000004000000ULL, 115659ULL,
+     {0xf8e9d8adfc5b9653ULL, 0xb8c7b117e7d8d566ULL, 0x00c805e26cadf5a9ULL, 0xb46553b17fef1432ULL, 0x4517810450a108b5ULL, 0x4c308878022e110bULL, 0x93019cb69192dfdfULL, 0xd98ebbbb270ec8ebULL, 0x65f56c9af042f1ebULL, 0x5d52787b79c28015ULL, 0xcd08b2f19f6f9e7fULL, 0xe45d09330461b52eULL, 0xeb57d3fa802e2803ULL, 0xe01404527ca951a6ULL, 0xa6a3cbb44312e850ULL, 0xb44f53280d3aa0f0ULL, 0xf128cb909f21b185ULL}},
+    {1000000000000000000ULL, 1000000000002000000ULL, 48427ULL,
+     {0x1725b31f655b0226ULL, 0x60a6ff72682a26d2ULL, 0xd8988e0b9135a43cULL, 0xd5969b80a8187475ULL, 0xd00694f6e6354bd0ULL, 0xd30b154c777dc267ULL, 0x16294824948c2a9dULL, 0x5038a6f0ce44ed96ULL, 0x2f0a1a1d1810fe42ULL, 0x1aff2fd0691a2dafULL, 0x31e094e7c60a99e8ULL, 0x44152bb55fda44c5ULL, 0x05f992d233d411b1ULL, 0xa62efd0e9552024eULL, 0x29cf767b857da8f3ULL, 0xcec9ecde3002d809ULL, 0x3b34bf9f7a2b6d93ULL}},
+    {9223372036852775808ULL, 9223372036854775808ULL, 45777ULL,
+     {0xff626de7fb316232ULL, 0x1d7d16db4165399bULL, 0xdc418aac8bcb404cULL, 0x0debd3cf6b884231ULL, 0x68a32de6eb7b7808ULL, 0xe9b4fe527ac8558dULL, 0xf53c6b79f162a8cbULL, 0x7cae99135f554f17ULL, 0x2c387a41f6ea09a5ULL, 0x629cbbb17a706ea3ULL, 0x99871c62d5032924ULL, 0xdbda0f3a9c6c30abULL, 0x5e92dd42e137b077ULL, 0x4ebd44f5dfd65eb9ULL, 0x00c12e131ea2b18cULL, 0xfd19e5fd0c9df246ULL, 0xf4ac8d4519009025ULL}},
+};
+
+/* Known Wieferich primes of the bases 2..30 that are not perfect powers (the OEIS values
+ * the reference's selftest checks, A001220/wieferich.c:699-723). */
+static const struct { u32 b; int n; u64 p[8]; } KNOWN_SOL[] = {
+    {2, 2, {1093ULL, 3511ULL}},
+    {3, 2, {11ULL, 1006003ULL}},
+    {5, 7, {2ULL, 20771ULL, 40487ULL, 53471161ULL, 1645333507ULL, 6692367337ULL, 188748146801ULL}},
+    {6, 3, {66161ULL, 534851ULL, 3152573ULL}},
+    {7, 2, {5ULL, 491531ULL}},
+    {10, 3, {3ULL, 487ULL, 56598313ULL}},
+    {11, 1, {71ULL}},
+    {12, 2, {2693ULL, 123653ULL}},
+    {13, 3, {2ULL, 863ULL, 1747591ULL}},
+    {14, 3, {29ULL, 353ULL, 7596952219ULL}},
+    {15, 2, {29131ULL, 119327070011ULL}},
+    {17, 5, {2ULL, 3ULL, 46021ULL, 48947ULL, 478225523351ULL}},
+    {18, 6, {5ULL, 7ULL, 37ULL, 331ULL, 33923ULL, 1284043ULL}},
+    {19, 6, {3ULL, 7ULL, 13ULL, 43ULL, 137ULL, 63061489ULL}},
+    {20, 4, {281ULL, 46457ULL, 9377747ULL, 122959073ULL}},
+    {21, 1, {2ULL}},
+    {22, 5, {13ULL, 673ULL, 1595813ULL, 492366587ULL, 9809862296159ULL}},
+    {23, 5, {13ULL, 2481757ULL, 13703077ULL, 15546404183ULL, 2549536629329ULL}},
+    {24, 2, {5ULL, 25633ULL}},
+    {26, 5, {3ULL, 5ULL, 71ULL, 486999673ULL, 6695256707ULL}},
+    {28, 3, {3ULL, 19ULL, 23ULL}},
+    {29, 1, {2ULL}},
+    {30, 4, {7ULL, 160541ULL, 94727075783ULL, 303632117562967ULL}},
+};
+#define NKNOWN_SOL ((int)(sizeof KNOWN_SOL / sizeof *KNOWN_SOL))
+
+static bool known_solution(u32 b, u64 p)
+{
+    for (int i = 0; i < NKNOWN_SOL; i++)
+        if (KNOWN_SOL[i].b == b)
+            for (int k = 0; k < KNOWN_SOL[i].n; k++) if (KNOWN_SOL[i].p[k] == p) return true;
+    return false;
+}
+
+static void use_bases(const char *list)
+{
+    parse_bases(list);
+    if (g_eng) {
+        int rc = wf_set_bases(g_eng, g_b, g_nb);
+        if (rc) die("wf_set_bases: %s", wf_strerror(rc));
+    }
+}
+
+/* selftest, the reference's three parts (wieferich.c:1821-1900): GPU arithmetic against
+ * shift-and-add, the "known" bases over [0, LIMIT) against the known solutions (and
+ * pi(10^9)), windows around the known solutions above LIMIT, and the checksum windows
+ * for the "oeis" bases (plus base 2). */
 static int cmd_selftest(int argc, char **argv)
 {
     u64 limit = 1000000000ULL;
@@ -844,16 +907,17 @@ static int cmd_selftest(int argc, char **argv)
         else if (argv[i][0] != '-') limit = parse_num(argv[i]);
         else usage();
     }
-    parse_bases("2");
+    use_bases("2");
     engine_open();
     int bad = 0;
     /* 0. GPU quotients of random odd numbers of every size against shift-and-add */
     {
-        enum { NQ = 20000 };
+        enum { NQ = 8000 };
         u64 *p = malloc(NQ * sizeof *p), *q = malloc(NQ * sizeof *q), *rh = malloc(NQ * sizeof *rh),
             *rl = malloc(NQ * sizeof *rl);
         const u64 edge[] = {3, 5, 7, 1093, 3511, 4294967291ULL, 4294967311ULL, (1ULL << 50) - 27, (1ULL << 63) - 25,
                             (1ULL << 63) + 29, 18446744073709551557ULL, UINT64_MAX};
+        const u32 tb[] = {2, 3, 30, 1625, 4294967291U};
         int n = 0;
         for (size_t i = 0; i < sizeof edge / sizeof *edge; i++) p[n++] = edge[i];
         while (n < NQ) {
@@ -861,25 +925,29 @@ static int cmd_selftest(int argc, char **argv)
             u64 x = (rnd() >> (64 - bits)) | (1ULL << (bits - 1)) | 1;
             if (x >= 3) p[n++] = x;
         }
-        int rc = wf_quotients(g_eng, p, q, rh, rl, NQ);
-        if (rc) die("wf_quotients: %s", wf_strerror(rc));
         int qbad = 0;
-        for (int i = 0; i < NQ && qbad < 5; i++) {
-            const u128 rr = powmod_ref(2, p[i] - 1, (u128)p[i] * p[i]);
-            u64 pinv = (3 * p[i]) ^ 2;                  /* p^-1 mod 2^64 by Newton */
-            for (int k = 0; k < 4; k++) pinv *= 2 - p[i] * pinv;
-            const u64 qr = ((u64)rr - 1) * pinv;
-            if ((((u128)rh[i] << 64) | rl[i]) != rr || q[i] != qr) {
-                qbad++;
-                fprintf(stderr, "  quotient mismatch p=%" PRIu64 "\n", p[i]);
+        for (size_t k = 0; k < sizeof tb / sizeof *tb; k++) {
+            int rc = wf_quotients_base(g_eng, tb[k], p, q, rh, rl, NQ);
+            if (rc) die("wf_quotients_base: %s", wf_strerror(rc));
+            for (int i = 0; i < NQ && qbad < 5; i++) {
+                const u128 rr = powmod_ref(tb[k], p[i] - 1, (u128)p[i] * p[i]);
+                u64 pinv = (3 * p[i]) ^ 2;                  /* p^-1 mod 2^64 by Newton */
+                for (int r = 0; r < 4; r++) pinv *= 2 - p[i] * pinv;
+                const u64 qr = ((u64)rr - 1) * pinv;
+                if ((((u128)rh[i] << 64) | rl[i]) != rr || q[i] != qr) {
+                    qbad++;
+                    fprintf(stderr, "  quotient mismatch base %u p=%" PRIu64 "\n", tb[k], p[i]);
+                }
             }
         }
-        fprintf(stderr, "arithmetic: %d GPU residues 2^(p-1) mod p^2 vs shift-and-add: %s\n", NQ, qbad ? "FAIL" : "ok");
+        fprintf(stderr, "arithmetic: %d GPU residues b^(p-1) mod p^2 for %zu bases vs shift-and-add: %s\n", NQ,
+                sizeof tb / sizeof *tb, qbad ? "FAIL" : "ok");
         bad += qbad;
         free(p); free(q); free(rh); free(rl);
     }
     g_print = false;
-    /* 1. [0, LIMIT): pi(10^9) and the base-2 solutions 1093, 3511 */
+    /* 1. the "known" bases over [0, LIMIT) against the known solutions */
+    use_bases("known");
     double t = now();
     u64 chunk = limit / 16;
     if (chunk < 1000000) chunk = 1000000;
@@ -887,35 +955,69 @@ static int cmd_selftest(int argc, char **argv)
     const bool pi_ok = limit != 1000000000ULL || S.primes == 50847534ULL;
     int found = 0, extra = 0, missing = 0;
     for (int i = 0; i < S.nhits; i++) {
-        if (S.hits[i].p == 1093 || S.hits[i].p == 3511) found++;
-        else { extra++; fprintf(stderr, "  unexpected solution p %" PRIu64 "\n", S.hits[i].p); }
+        if (known_solution(S.hits[i].b, S.hits[i].p)) found++;
+        else { extra++; fprintf(stderr, "  unexpected solution base %u p %" PRIu64 "\n", S.hits[i].b, S.hits[i].p); }
     }
-    const u64 known[] = {1093, 3511};
-    for (int k = 0; k < 2; k++) {
-        if (known[k] >= (u64)S.end) continue;
-        bool f = false;
-        for (int i = 0; i < S.nhits; i++) if (S.hits[i].p == known[k]) f = true;
-        if (!f) { missing++; fprintf(stderr, "  missing p %" PRIu64 "\n", known[k]); }
-    }
-    fprintf(stderr, "scan [0, %" PRIu64 "), base 2: %" PRIu64 " primes%s, %d known solutions found, %d missing, %d unexpected, %" PRIu64
-                    " FLT errors (%.1f s)\n", (u64)S.end, S.primes,
+    for (int i = 0; i < NKNOWN_SOL; i++)
+        for (int k = 0; k < KNOWN_SOL[i].n; k++) {
+            if (KNOWN_SOL[i].p[k] >= (u64)S.end) continue;
+            bool f = false;
+            for (int h = 0; h < S.nhits; h++) if (S.hits[h].b == KNOWN_SOL[i].b && S.hits[h].p == KNOWN_SOL[i].p[k]) f = true;
+            if (!f) { missing++; fprintf(stderr, "  missing base %u p %" PRIu64 "\n", KNOWN_SOL[i].b, KNOWN_SOL[i].p[k]); }
+        }
+    fprintf(stderr, "scan [0, %" PRIu64 "), %d bases: %" PRIu64 " primes%s, %d known solutions found, %d missing, %d unexpected, %" PRIu64
+                    " FLT errors (%.1f s)\n", (u64)S.end, g_nb, S.primes,
             limit == 1000000000ULL ? (pi_ok ? " (= pi(10^9))" : " (WRONG, pi(10^9) = 50847534)") : "", found, missing,
             extra, S.flt, now() - t);
     bad += missing + extra + (int)S.flt + !pi_ok;
-    /* 2. checksum windows */
+    /* 2. windows around the known solutions above the scan */
+    t = now();
+    const u64 scanned = (u64)S.end;
+    int wins = 0, wbad = 0;
+    for (int i = 0; i < NKNOWN_SOL; i++)
+        for (int k = 0; k < KNOWN_SOL[i].n; k++) {
+            const u64 p = KNOWN_SOL[i].p[k];
+            if (p < scanned) continue;
+            char one[16];
+            snprintf(one, sizeof one, "%u", KNOWN_SOL[i].b);
+            use_bases(one);
+            run_scan(p > 1000000 ? p - 1000000 : 0, p + 1000000, 200000, NULL, 60, true);
+            const bool ok = S.nhits == 1 && S.hits[0].p == p && S.flt == 0;
+            if (!ok) { wbad++; fprintf(stderr, "  window around base %u p %" PRIu64 ": %d hits\n", KNOWN_SOL[i].b, p, S.nhits); }
+            wins++;
+        }
+    fprintf(stderr, "windows: %d known solutions above %" PRIu64 " re-found through the scan: %s (%.1f s)\n", wins,
+            scanned, wbad ? "FAIL" : "ok", now() - t);
+    bad += wbad;
+    /* 3. checksum windows: the "oeis" bases against wieferich_cks.h, base 2 against Python integers */
     t = now();
     int cbad = 0;
+    use_bases("oeis");
+    for (size_t w = 0; w < sizeof CKWIN_OEIS / sizeof *CKWIN_OEIS; w++) {
+        run_scan(CKWIN_OEIS[w].lo, CKWIN_OEIS[w].hi, (CKWIN_OEIS[w].hi - CKWIN_OEIS[w].lo) / 16, NULL, 60, true);
+        bool ok = S.primes == CKWIN_OEIS[w].primes && S.flt == 0;
+        for (int j = 0; j < g_nb; j++) if (S.cks[j] != CKWIN_OEIS[w].cks[j]) ok = false;
+        if (!ok) {
+            cbad++;
+            fprintf(stderr, "  checksum window [%" PRIu64 ", %" PRIu64 "): primes %" PRIu64 "\n", CKWIN_OEIS[w].lo,
+                    CKWIN_OEIS[w].hi, S.primes);
+            for (int j = 0; j < g_nb; j++)
+                if (S.cks[j] != CKWIN_OEIS[w].cks[j])
+                    fprintf(stderr, "    base %u: %016" PRIx64 " expected %016" PRIx64 "\n", g_b[j], S.cks[j], CKWIN_OEIS[w].cks[j]);
+        }
+    }
+    use_bases("2");
     for (size_t w = 0; w < sizeof CKWIN2 / sizeof *CKWIN2; w++) {
         run_scan(CKWIN2[w].lo, CKWIN2[w].hi, (CKWIN2[w].hi - CKWIN2[w].lo) / 16, NULL, 60, true);
         const bool ok = S.primes == CKWIN2[w].primes && S.flt == 0 && S.cks[0] == CKWIN2[w].cks;
         if (!ok) {
             cbad++;
-            fprintf(stderr, "  checksum window [%" PRIu64 ", %" PRIu64 "): primes %" PRIu64 " cks %016" PRIx64 "\n",
+            fprintf(stderr, "  checksum window [%" PRIu64 ", %" PRIu64 "), base 2: primes %" PRIu64 " cks %016" PRIx64 "\n",
                     CKWIN2[w].lo, CKWIN2[w].hi, S.primes, S.cks[0]);
         }
     }
-    fprintf(stderr, "checksums: %zu windows, base 2, vs Python integers: %s (%.1f s)\n", sizeof CKWIN2 / sizeof *CKWIN2,
-            cbad ? "FAIL" : "ok", now() - t);
+    fprintf(stderr, "checksums: %zu windows x 17 'oeis' bases (wieferich_cks.h) and base 2 (Python integers): %s (%.1f s)\n",
+            sizeof CKWIN_OEIS / sizeof *CKWIN_OEIS, cbad ? "FAIL" : "ok", now() - t);
     bad += cbad;
     fprintf(stderr, "selftest %s\n", bad ? "FAILED" : "passed");
     wf_close(g_eng);

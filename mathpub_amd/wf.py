@@ -68,6 +68,7 @@ def load():
     L.wf_scan_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Range), u32, i64, ctypes.POINTER(Result),
                                 ctypes.POINTER(Event), u64]
     L.wf_quotients.argtypes = [ctypes.c_void_p, P64, P64, P64, P64, u64]
+    L.wf_quotients_base.argtypes = [ctypes.c_void_p, u32, P64, P64, P64, P64, u64]
     L.wf_primes.argtypes = [ctypes.c_void_p, u64, u64, P64, u64, P64]
     L.wf_last_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(Timing)]
     L.wf_autotune.argtypes = [ctypes.c_void_p, u64, u64, ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_double)]
@@ -75,7 +76,8 @@ def load():
     L.wf_serve_submit.argtypes = [ctypes.c_void_p, u64, u64, P64]
     L.wf_serve_poll.argtypes = [ctypes.c_void_p, u64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_int)]
     L.wf_serve_stop.argtypes = [ctypes.c_void_p]
-    for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_primes",
+    for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_quotients_base",
+              "wf_primes",
               "wf_last_timing", "wf_autotune", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
         getattr(L, f).restype = ctypes.c_int
     _lib = L
@@ -95,10 +97,12 @@ def _check(rc, what):
         raise WFError(rc, what)
 
 
-def result_dict(res, events=None):
+def result_dict(res, events=None, nb=1):
+    """Totals of one range: cks / sumq of the first base, cks_all / sumq_all of every base in
+    the engine's order; events as (type, p, q, r, base) sorted by (p, base)."""
     return {"primes": res.primes, "tests": res.tests, "near": res.near, "flt": res.flt,
-            "cks": res.cks[0], "sumq": res.sumq[0], "nev": res.nev, "npsp": res.npsp,
-            "events": events if events is not None else []}
+            "cks": res.cks[0], "sumq": res.sumq[0], "cks_all": list(res.cks[:nb]), "sumq_all": list(res.sumq[:nb]),
+            "nev": res.nev, "npsp": res.npsp, "events": events if events is not None else []}
 
 
 class Engine:
@@ -111,6 +115,7 @@ class Engine:
         _check(L.wf_open(device, ctypes.byref(cfg), ctypes.byref(h)), "wf_open")
         self._h = h
         self._L = L
+        self._bases = [2]
 
     def close(self):
         if self._h:
@@ -135,8 +140,13 @@ class Engine:
         return buf.value.decode()
 
     def set_bases(self, bases):
+        """Bases of the run, ascending and distinct (wf_set_bases)."""
         arr = (u32 * len(bases))(*bases)
         _check(self._L.wf_set_bases(self._h, arr, len(bases)), "wf_set_bases")
+        self._bases = list(bases)
+
+    def _events(self, ev, n):
+        return [(e.type, e.p, e.q, (e.r_hi << 64) | e.r_lo, self._bases[e.j]) for e in ev[:n]]
 
     def scan(self, lo, hi, near_T=0, ev_cap=1 << 16, psp=False, psp_cap=1 << 16):
         res = Result()
@@ -145,7 +155,7 @@ class Engine:
         _check(self._L.wf_scan(self._h, lo, _hi(hi), near_T, ctypes.byref(res), ev, ev_cap, 1 if psp else 0,
                                pbuf, psp_cap), "wf_scan")
         n = min(res.nev, ev_cap)
-        out = result_dict(res, [(e.type, e.p, e.q, (e.r_hi << 64) | e.r_lo) for e in ev[:n]])
+        out = result_dict(res, self._events(ev, n), len(self._bases))
         out["psp"] = list(pbuf[:min(res.npsp, psp_cap)]) if psp else None
         return out
 
@@ -156,14 +166,15 @@ class Engine:
         ev = (Event * max(ev_cap, 1))()
         _check(self._L.wf_scan_batch(self._h, rr, n, near_T, res, ev, ev_cap), "wf_scan_batch")
         total_ev = sum(r.nev for r in res)
-        events = [(e.type, e.p, e.q, (e.r_hi << 64) | e.r_lo) for e in ev[:min(total_ev, ev_cap)]]
-        return [result_dict(r) for r in res], events
+        events = self._events(ev, min(total_ev, ev_cap))
+        return [result_dict(r, None, len(self._bases)) for r in res], events
 
-    def quotients(self, ps):
+    def quotients(self, ps, base=2):
+        """[(q, r)] with r = base^(p-1) mod p^2, q = (r - 1) p^-1 mod 2^64 (wf_quotients_base)."""
         n = len(ps)
         arr = (u64 * n)(*ps)
         q, rh, rl = (u64 * n)(), (u64 * n)(), (u64 * n)()
-        _check(self._L.wf_quotients(self._h, arr, q, rh, rl, n), "wf_quotients")
+        _check(self._L.wf_quotients_base(self._h, base, arr, q, rh, rl, n), "wf_quotients_base")
         return [(q[i], (rh[i] << 64) | rl[i]) for i in range(n)]
 
     def primes(self, lo, hi):
@@ -197,7 +208,7 @@ class Engine:
         """Result dict once the request is done, else None."""
         res, done = Result(), ctypes.c_int()
         _check(self._L.wf_serve_poll(self._h, ticket, ctypes.byref(res), ctypes.byref(done)), "wf_serve_poll")
-        return result_dict(res) if done.value else None
+        return result_dict(res, None, len(self._bases)) if done.value else None
 
     def serve_stop(self):
         _check(self._L.wf_serve_stop(self._h), "wf_serve_stop")

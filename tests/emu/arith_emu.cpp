@@ -40,6 +40,52 @@ void emu_mont(int L, int full, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo
 }
 
 
+// window and powers of a base, as the engine sets them up: the largest w <= 5 with
+// b^(2^w - 1) < 2^32, pw[d] = b^d
+static int base_table(uint32_t b, uint32_t* pw) {
+  int w = 5;
+  for (;;) {
+    unsigned __int128 x = 1;
+    bool ok = true;
+    for (int k = 1; k < (1 << w); k++) { x *= b; if (x >> 32) { ok = false; break; } }
+    if (ok || w == 1) break;
+    w--;
+  }
+  uint64_t x = 1;
+  for (int d = 0; d < (1 << w); d++) { pw[d] = (uint32_t)x; x *= b; }
+  return w;
+}
+
+void emu_fp64c(uint32_t base, uint64_t p0, uint64_t p1, int ebits, uint64_t* r) {
+  uint32_t pw[32];
+  const int w = base_table(base, pw);
+  double a0, b0, a1, b1;
+  switch (w) {
+    case 5: fp64_powc_x2<5>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
+    case 4: fp64_powc_x2<4>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
+    case 3: fp64_powc_x2<3>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
+    case 2: fp64_powc_x2<2>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
+    default: fp64_powc_x2<1>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
+  }
+  const unsigned __int128 r0 = (unsigned __int128)(uint64_t)b0 * p0 + (uint64_t)a0;
+  const unsigned __int128 r1 = (unsigned __int128)(uint64_t)b1 * p1 + (uint64_t)a1;
+  r[0] = (uint64_t)(r0 >> 64); r[1] = (uint64_t)r0; r[2] = (uint64_t)(r1 >> 64); r[3] = (uint64_t)r1;
+}
+
+void emu_m28c(uint32_t base, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
+  uint32_t pw[32], x[4];
+  const int w = base_table(base, pw);
+  switch (w) {
+    case 5: m28_powc<5>(p, p - 1, ebits, pw, x); break;
+    case 4: m28_powc<4>(p, p - 1, ebits, pw, x); break;
+    case 3: m28_powc<3>(p, p - 1, ebits, pw, x); break;
+    case 2: m28_powc<2>(p, p - 1, ebits, pw, x); break;
+    default: m28_powc<1>(p, p - 1, ebits, pw, x); break;
+  }
+  *lo = (uint64_t)x[0] | ((uint64_t)x[1] << 32);
+  *hi = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
+}
+
 int emu_sprp2(uint64_t n) { return sprp2_64(n) ? 1 : 0; }
 
 int emu_psp_possible(uint64_t n) { return psp_possible(n) ? 1 : 0; }

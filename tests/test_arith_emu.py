@@ -27,6 +27,8 @@ def emu():
     L.emu_psp_possible.argtypes = [ctypes.c_uint64]
     L.emu_m28.argtypes = [ctypes.c_uint64, ctypes.c_int, P, P]
     L.emu_fp64x2.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
+    L.emu_fp64c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
+    L.emu_m28c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, P, P]
     return L
 
 
@@ -174,3 +176,40 @@ def _mr(n, a):
         if x == n - 1:
             return True
     return False
+
+
+BASES = [2, 3, 5, 6, 7, 10, 12, 13, 14, 15, 17, 18, 19, 20, 22, 23, 26, 30, 31, 1625, 1626, 65535, 65536,
+         (1 << 31) - 1, (1 << 32) - 1]
+
+
+def test_fp64_any_base(emu):
+    """Any base 2 <= b < 2^32 through the FP64 two-ladder form (window 5..1 by base), p < 2^50,
+    including p | b pairs (r = 0) and small p below the multiplier."""
+    rng = random.Random(21)
+    ps = [3, 5, 7, 11, 13, 1093, 3511, (1 << 50) - 27] + samples(2, 50, 1500, 8)
+    out = (ctypes.c_uint64 * 4)()
+    for b in BASES:
+        for p0 in ps:
+            if p0 < 3:
+                continue
+            p1 = rng.choice(ps) | 1
+            if p1 < 3:
+                p1 = 3
+            eb = max((p0 - 1).bit_length(), (p1 - 1).bit_length())
+            eb = min(50, eb + rng.randint(0, 3))
+            emu.emu_fp64c(b, p0, p1, eb, out)
+            assert (out[0] << 64 | out[1]) == pow(b, p0 - 1, p0 * p0), (b, p0)
+            assert (out[2] << 64 | out[3]) == pow(b, p1 - 1, p1 * p1), (b, p1)
+
+
+def test_radix28_any_base(emu):
+    """Any base through the radix-2^28 ladder with small-constant multiplication, 2^50 <= p < 2^64."""
+    ps = [(1 << 50) + 27, (1 << 63) - 25, (1 << 63) + 29, (1 << 64) - 59, (1 << 64) - 1, 10 ** 18 + 9] + \
+        samples(50, 64, 800, 9)
+    rng = random.Random(22)
+    for b in BASES:
+        for p in ps:
+            if p < (1 << 50):
+                continue
+            eb = min(64, (p - 1).bit_length() + rng.randint(0, 2))
+            assert call(emu.emu_m28c, b, p, eb) == pow(b, p - 1, p * p), (b, p)

@@ -73,7 +73,7 @@ def write_ckpt(path, **kw):
 def test_checkpoint_validation(tmp_path, kw, msg):
     ck = tmp_path / "st"
     write_ckpt(ck, **kw)
-    rc, out, err = run("scan", "1000", "2000", "-c", "100", "-S", ck)
+    rc, out, err = run("scan", "1000", "2000", "-c", "100", "-b", "2", "-S", ck)
     assert rc == 1 and msg in err, err
 
 
@@ -88,20 +88,21 @@ def test_foreign_file_rejected(tmp_path):
 # ---- on the GPU ----
 
 def parse_summary(out):
+    """SUMMARY totals and the per-base checksum lines ((base, cks), ...)."""
     m = re.search(r"SUMMARY \[(\d+), (\d+)\) primes (\d+) tests (\d+) flt_errors (\d+)", out)
-    cks = re.search(r"base   2 A001220  cks ([0-9a-f]{16})", out)
-    return m.groups(), cks.group(1)
+    cks = tuple(re.findall(r"^base +(\d+) +(?:A\d+ +)?cks ([0-9a-f]{16})", out, re.M))
+    return m.groups(), cks
 
 
 @pytest.mark.gpu
 def test_scan_matches_oracle_and_log_format(tmp_path):
     import oracle
     log = tmp_path / "log"
-    rc, out, err = run("scan", "1e15", "1e15+3e8", "-c", "1e8", "-n", "1000", "-L", log, "-q", "-Q")
+    rc, out, err = run("scan", "1e15", "1e15+3e8", "-c", "1e8", "-b", "2", "-n", "1000", "-L", log, "-q", "-Q")
     assert rc == 0, err
     (lo, hi, primes, tests, flt), cks = parse_summary(out)
     o = oracle.scan(10 ** 15, 10 ** 15 + 3 * 10 ** 8, bases=[2], near_T=1000)
-    assert int(primes) == o["primes"] and cks == "%016x" % o["cks"][2] and int(flt) == 0
+    assert int(primes) == o["primes"] and cks == (("2", "%016x" % o["cks"][2]),) and int(flt) == 0
     assert "sumq base 2 %016x" % o["sumq"][2] in out
     lines = log.read_text().splitlines()
     assert len(lines) == 3
@@ -118,7 +119,7 @@ def test_scan_matches_oracle_and_log_format(tmp_path):
 
 @pytest.mark.gpu
 def test_found_and_confirmed_lines():
-    rc, out, err = run("scan", "0", "1e5", "-c", "2e4", "-q")
+    rc, out, err = run("scan", "0", "1e5", "-c", "2e4", "-b", "2", "-q")
     assert rc == 0
     assert "FOUND base 2 p 1093 (A001220)" in out and "FOUND base 2 p 3511 (A001220)" in out
     assert "CONFIRMED base 2 p 1093 (A001220) (all primes in [0, 20000) tested)" in out
@@ -128,7 +129,7 @@ def test_found_and_confirmed_lines():
 @pytest.mark.gpu
 def test_resume_equals_straight_run(tmp_path):
     ck = tmp_path / "st"
-    straight = run("scan", "1e12", "1e12+4e8", "-c", "5e7", "-q")[1]
+    straight = run("scan", "1e12", "1e12+4e8", "-c", "5e7", "-q")[1]     # default bases: oeis
     r1 = run("scan", "1e12", "1e12+2e8", "-c", "5e7", "-q", "-S", ck)
     assert r1[0] == 0 and ck.exists()
     r2 = run("scan", "1e12", "1e12+4e8", "-c", "5e7", "-q", "-S", ck)      # END may change on resume
@@ -139,15 +140,15 @@ def test_resume_equals_straight_run(tmp_path):
 @pytest.mark.gpu
 def test_sigint_saves_checkpoint_then_resumes(tmp_path):
     ck = tmp_path / "st"
-    straight = run("scan", "1e15", "1e15+3e10", "-c", "1e9", "-q")[1]
-    p = subprocess.Popen([WF, "scan", "1e15", "1e15+3e10", "-c", "1e9", "-S", str(ck)], stdout=subprocess.PIPE,
+    straight = run("scan", "1e15", "1e15+3e10", "-c", "1e9", "-b", "2", "-q")[1]
+    p = subprocess.Popen([WF, "scan", "1e15", "1e15+3e10", "-c", "1e9", "-b", "2", "-S", str(ck)], stdout=subprocess.PIPE,
                          stderr=subprocess.PIPE, text=True)
     time.sleep(3.0)
     p.send_signal(signal.SIGINT)
     out, err = p.communicate(timeout=300)
     assert p.returncode == 0 and ck.exists()
     assert "stopped at position" in err or "SUMMARY [1000000000000000, 1000030000000000)" in out
-    r2 = run("scan", "1e15", "1e15+3e10", "-c", "1e9", "-q", "-S", ck)
+    r2 = run("scan", "1e15", "1e15+3e10", "-c", "1e9", "-b", "2", "-q", "-S", ck)
     assert parse_summary(r2[1]) == parse_summary(straight)
 
 
@@ -165,3 +166,34 @@ def test_check_command():
 def test_selftest_small():
     rc, out, err = run("selftest", "1e8", timeout=900)
     assert rc == 0 and "selftest passed" in err, err
+
+
+OEIS = [3, 5, 6, 7, 10, 12, 13, 14, 15, 17, 18, 19, 20, 22, 23, 26, 30]
+
+
+@pytest.mark.gpu
+def test_default_oeis_bases_match_oracle(tmp_path):
+    """The reference's default base list ("oeis", 17 bases): SUMMARY checksums and every
+    chunk-log line agree with the oracle, byte for byte."""
+    import oracle
+    log = tmp_path / "log"
+    rc, out, err = run("scan", "1e12", "1e12+2e8", "-c", "1e8", "-n", "100", "-L", log, "-q")
+    assert rc == 0, err
+    (lo, hi, primes, tests, flt), cks = parse_summary(out)
+    o = oracle.scan(10 ** 12, 10 ** 12 + 2 * 10 ** 8, bases=OEIS, near_T=100)
+    assert int(primes) == o["primes"] and int(tests) == o["tests"] and int(flt) == 0
+    assert cks == tuple((str(b), "%016x" % o["cks"][b]) for b in OEIS)
+    for k, ln in enumerate(log.read_text().splitlines()):
+        a = 10 ** 12 + k * 10 ** 8
+        oc = oracle.scan(a, a + 10 ** 8, bases=OEIS)
+        assert ln == "C %d %d %d " % (a, a + 10 ** 8, oc["primes"]) + " ".join(
+            "%d:%016x" % (b, oc["cks"][b]) for b in OEIS)
+    near = [l for l in out.splitlines() if l.startswith("NEAR")]
+    assert len(near) == o["near"]
+
+
+@pytest.mark.gpu
+def test_check_known_bases():
+    rc, out, err = run("check", "1006003")
+    assert rc == 0 and "base   3: q = 0  A = 0  <-- WIEFERICH  [reference agrees]" in out, out
+    assert "DISAGREES" not in out

@@ -30,7 +30,7 @@ def eng(request):
 
 
 def near_of(events):
-    return [[p, (q if q <= p // 2 else q - p)] for t, p, q, r in events if t == wf.EV_NEAR]
+    return [[p, (q if q <= p // 2 else q - p)] for t, p, q, r, b in events if t == wf.EV_NEAR]
 
 
 @pytest.mark.parametrize("lo,hi", [(0, 1000), (0, 3), (2, 3), (3, 4), (7, 8), (1, 120), (29, 31), (30, 31),
@@ -71,7 +71,7 @@ def test_golden_windows(eng, w):
     assert "%016x" % r["sumq"] == g["sumq2"]
     assert r["flt"] == 0
     assert near_of(r["events"]) == g["near"]
-    assert [p for t, p, q, rr in r["events"] if t == wf.EV_FOUND] == g["hits"]
+    assert [p for t, p, q, rr, b in r["events"] if t == wf.EV_FOUND] == g["hits"]
     assert r["tests"] == r["primes"] - (1 if g["lo"] <= 2 < g["hi"] else 0)
 
 
@@ -83,14 +83,14 @@ def test_scan_vs_oracle(eng, lo, hi):
     o = oracle.scan(lo, hi, bases=[2], near_T=1000)
     assert (r["primes"], r["tests"], r["near"], r["flt"]) == (o["primes"], o["tests"], o["near"], o["flt"])
     assert r["cks"] == o["cks"][2] and r["sumq"] == o["sumq"][2]
-    assert [(p, q) for t, p, q, rr in r["events"]] == [(e[2], e[3]) for e in o["events"]]
+    assert [(p, q) for t, p, q, rr, b in r["events"]] == [(e[2], e[3]) for e in o["events"]]
 
 
 def test_c1_pi_1e9(eng):
     say("C1 [3, 1e9)")
     r = eng.scan(3, 10 ** 9, near_T=100)
     assert r["primes"] == FACTS["pi_1e9"] - 1
-    assert [p for t, p, q, rr in r["events"] if t == wf.EV_FOUND] == [1093, 3511]
+    assert [p for t, p, q, rr, b in r["events"] if t == wf.EV_FOUND] == [1093, 3511]
     o = oracle.scan(3, 10 ** 9, bases=[2], near_T=100)
     assert r["cks"] == o["cks"][2] and r["sumq"] == o["sumq"][2] and r["near"] == o["near"]
 
@@ -202,3 +202,90 @@ def test_tiny_and_empty_ranges(eng):
     for (lo, hi), r in zip(cases, per):
         o = oracle.scan(lo, hi, bases=[2])
         assert (r["primes"], r["cks"]) == (o["primes"], o["cks"][2]), (lo, hi)
+
+
+# ---- any base (the reference's -b lists) ----
+
+OEIS = FACTS["oeis_bases"]
+
+
+@pytest.fixture(scope="module")
+def feng():
+    e = wf.Engine()
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("w", range(3))
+def test_ckwin_17_bases_reference_data(feng, w):
+    """The reference's own checksum windows (wieferich_cks.h, generated by verify_wieferich.py)
+    for its 17 default bases: FP64 below 2^50, radix-2^28 above."""
+    c = FACTS["ckwin"][w]
+    say("ckwin", c["lo"], c["hi"])
+    feng.set_bases(OEIS)
+    r = feng.scan(c["lo"], c["hi"])
+    assert r["primes"] == c["primes"] and r["flt"] == 0
+    assert ["%016x" % x for x in r["cks_all"]] == c["cks"]
+
+
+def test_known_solutions_all_bases(feng):
+    """Every known solution of the 23 'known' bases below 10^8, p = 2 for b = 1 (mod 4)
+    included, and nothing else (OEIS values the reference's selftest checks)."""
+    known = {int(b): v for b, v in FACTS["known"].items()}
+    bases = sorted(known)
+    feng.set_bases(bases)
+    r = feng.scan(0, 10 ** 8)
+    got = sorted((b, p) for t, p, q, rr, b in r["events"] if t == wf.EV_FOUND)
+    want = sorted((b, p) for b in bases for p in known[b] if p < 10 ** 8)
+    assert got == want
+    assert r["primes"] == 5761455 and r["flt"] == 0
+
+
+@pytest.mark.parametrize("lo,hi", [(3, 2 * 10 ** 6), (10 ** 12, 10 ** 12 + 2 * 10 ** 7),
+                                   ((1 << 50) - 5 * 10 ** 6, (1 << 50) + 5 * 10 ** 6),
+                                   ((1 << 63) - 5 * 10 ** 6, (1 << 63) + 5 * 10 ** 6), ((1 << 64) - 10 ** 7, 1 << 64)])
+def test_any_bases_vs_oracle(feng, lo, hi):
+    """Windows 5..1 (bases up to 2^32 - 1), p | b pairs, near lists and sums of q per base."""
+    bases = [2, 3, 6, 10, 24, 30, 97, 1625, 1626, 65537, 4294967291]
+    feng.set_bases(bases)
+    say("bases vs oracle", lo, hi)
+    r = feng.scan(lo, hi, near_T=100)
+    o = oracle.scan(lo, hi, bases=bases, near_T=100)
+    assert (r["primes"], r["tests"], r["near"], r["flt"]) == (o["primes"], o["tests"], o["near"], o["flt"])
+    assert r["cks_all"] == [o["cks"][b] for b in bases]
+    assert r["sumq_all"] == [o["sumq"][b] for b in bases]
+    assert [(b, p, q) for t, p, q, rr, b in r["events"]] == [(e[1], e[2], e[3]) for e in o["events"]]
+
+
+def test_quotients_any_base(feng):
+    rng = random.Random(31)
+    ps = [3, 5, 7, 1093, (1 << 50) - 27, (1 << 63) + 29, (1 << 64) - 59] + \
+        [rng.getrandbits(rng.randint(2, 64)) | 1 for _ in range(2000)]
+    ps = [p for p in ps if p >= 3]
+    for b in (3, 30, 1625, 65537, 4294967291):
+        got = feng.quotients(ps, base=b)
+        for p, (q, r) in zip(ps, got):
+            assert r == pow(b, p - 1, p * p), (b, p)
+
+
+def test_batch_any_bases_equals_single(feng):
+    feng.set_bases(OEIS)
+    rng = random.Random(8)
+    ranges = [(lo, lo + rng.randrange(1, 2 * 10 ** 6)) for lo in (rng.randrange(10 ** 9, 10 ** 18) for _ in range(12))]
+    ranges += [(0, 1000), ((1 << 50) - 100, (1 << 50) + 100)]
+    per, ev = feng.scan_batch(ranges)
+    for (lo, hi), r in zip(ranges, per):
+        s = feng.scan(lo, hi)
+        assert (r["primes"], r["tests"], r["cks_all"], r["sumq_all"]) == (s["primes"], s["tests"], s["cks_all"],
+                                                                         s["sumq_all"]), (lo, hi)
+
+
+def test_int32_path_is_base_2_only():
+    e = wf.Engine(arith=1)
+    try:
+        e.set_bases([2, 3])
+        with pytest.raises(wf.WFError) as ex:
+            e.scan(10 ** 12, 10 ** 12 + 1000)
+        assert ex.value.rc == -4
+    finally:
+        e.close()
