@@ -173,6 +173,41 @@ def bench_c4(args):
                          "test_ms": tm["test_ms"]})
 
 
+OEIS_BASES = [3, 5, 6, 7, 10, 12, 13, 14, 15, 17, 18, 19, 20, 22, 23, 26, 30]
+# The reference's published speed table (A001220/README.md:116-120): tests/s on an NVIDIA GB10
+# for the 16-17 bases at 2e14 and for base 3 at 1.2e15.
+PUBLISHED = {"oeis": {"lo": 2 * 10 ** 14, "bases": OEIS_BASES, "gb10": 8.3e8,
+                      "metric": "tests/s (prime, base pairs), the 17 'oeis' bases at p~2e14"},
+             "base3": {"lo": 12 * 10 ** 14, "bases": [3], "gb10": 7.2e8, "metric": "tests/s, base 3 at p~1.2e15"}}
+
+
+def bench_published(args):
+    """The two workloads of the reference's published speed table, same bases and heights:
+    vs_baseline is this engine's rate over the reference GPU tool's (NVIDIA GB10)."""
+    from mathpub_amd import wf
+    w = PUBLISHED[args.workload]
+    lo, span = w["lo"], int(args.span)
+    eng = wf.Engine()
+    eng.set_bases(w["bases"])
+    tuned = None if args.no_tune else eng.autotune(lo, lo + span)
+    for _ in range(args.warmup):
+        eng.scan_batch([(lo, lo + span)])
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        per, ev = eng.scan_batch([(lo, lo + span)])
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    r = per[0]
+    eng.close()
+    value = r["tests"] / (ms / 1e3)
+    return _engine_line(w["metric"], value, "tests/s", ms, args,
+                        {"workload": "[%.3g, %.3g + %.3g), bases %s" % (lo, lo, span, ",".join(map(str, w["bases"]))),
+                         "engine": "batched", "primes_per_step": r["primes"], "tests_per_step": r["tests"],
+                         "cks": ["%016x" % c for c in r["cks_all"]], "autotuned": tuned,
+                         "published_gb10_tests_per_s": w["gb10"], "published_source": "A001220/README.md:116-120"},
+                        {"vs_baseline": value / w["gb10"], "dtype": "f64",
+                         "data": "synthetic (the range is the input; primes sieved on the GPU)"})
+
+
 def c5_queries(n, seed=0x243F6A8885A308D3 ^ 5):
     """n range queries: lengths log-uniform in [1e5, 1e10], starts uniform in [1e9, 1e18]."""
     import random
@@ -244,9 +279,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
     ap.add_argument("--span", type=float, default=SPAN, help="numbers per step (default 1e11)")
     ap.add_argument("--mode", choices=["both", "plain", "batched"], default="both")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "oeis", "base3"], default="c2",
                     help="c2 = the single range (default, the headline line); c3 = 4096 batched ranges; "
-                         "c4 = a slice of the top-of-word range with strong-pseudoprime flags; c5 = serving")
+                         "c4 = a slice of the top-of-word range with strong-pseudoprime flags; c5 = serving; "
+                         "oeis / base3 = the reference's published speed workloads (17 bases at 2e14, base 3 at 1.2e15)")
     ap.add_argument("--queries", type=int, default=2000, help="c5: queries per rate")
     ap.add_argument("--rates", default="100,1000,5000", help="c5: Poisson arrival rates per second")
     ap.add_argument("--c4-span", type=float, default=2.0 ** 34, help="c4: numbers below 2^64 per step")
@@ -256,7 +292,8 @@ def main():
                     help="batched: test workgroups per CU (0 = engine default)")
     args = ap.parse_args()
     if args.workload != "c2" and args.gpus == 1:
-        return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
+        return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "oeis": bench_published,
+                "base3": bench_published}[args.workload](args)
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
               "(ranges shard by splitting [lo, hi), see DESIGN.md)", file=sys.stderr)
