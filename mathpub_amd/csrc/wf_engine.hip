@@ -217,6 +217,37 @@ __device__ __forceinline__ bool first_offset(u64 base, u32 q, u64 minv, u64 span
   return o < span;
 }
 
+// Every hit of an item's primes in its fill group: hit(segment of the group, byte offset
+// within the run, class).  A lane takes one prime (all 8 classes) or, when the slice is
+// short, one (prime, class) pair; primes with few hits per run walk their multiples.
+template <class F>
+__device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __restrict__ fruns,
+                                          const u32* __restrict__ sp, const u64* __restrict__ spinv,
+                                          const WheelLds& wl, int sbshift, bool pairs, F hit) {
+  const u32 nwork = pairs ? (it.i1 - it.i0) * 8 : it.i1 - it.i0;
+  for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
+    const u32 i = it.i0 + (pairs ? w >> 3 : w);
+    const u32 q = sp[i];
+    const u64 minv = spinv[i];
+    const int qi = wl.qi[q % 30];
+    const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
+    for (u32 r = 0; r < it.nfr; r++) {
+      const FillRun fr = fruns[it.fr0 + r];
+      const u64 nbytes = (u64)fr.nseg << sbshift;
+      const u64 span = 30ull * nbytes;
+      u64 o;
+      if (!first_offset(fr.base, q, minv, span, o)) continue;
+      if (!pairs && (u64)WALK_DIV * q > span) {
+        walk_multiples(o, q, span, wl, [&](u32 j, int c) { hit(fr.lseg0 + (j >> sbshift), j, (u32)c); });
+        continue;
+      }
+      for (int c = c0; c < c1; c++)
+        for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q)
+          hit(fr.lseg0 + (u32)(j >> sbshift), (u32)j, (u32)c);
+    }
+  }
+}
+
 // Large primes -> bucket records.  A work item covers a slice of the large primes over one
 // fill group: consecutive chunks of segments (of one range, or of several short ranges),
 // RUN_MAX segments at most, sized by the host to about FILL_TARGET expected records.  A
@@ -249,28 +280,8 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
   __syncthreads();
   // few primes with many hits each: one lane per (prime, class); else one lane per prime
   const bool pairs = it.i1 - it.i0 < blockDim.x;
-  const u32 nwork = pairs ? (it.i1 - it.i0) * 8 : it.i1 - it.i0;
-  for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
-    const u32 i = it.i0 + (pairs ? w >> 3 : w);
-    const u32 q = sp[i];
-    const u64 minv = spinv[i];
-    const int qi = wl.qi[q % 30];
-    const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
-    for (u32 r = 0; r < it.nfr; r++) {
-      const FillRun fr = fruns[it.fr0 + r];
-      const u64 nbytes = (u64)fr.nseg << sbshift;
-      const u64 span = 30ull * nbytes;
-      u64 o;
-      if (!first_offset(fr.base, q, minv, span, o)) continue;
-      if (!pairs && (u64)WALK_DIV * q > span) {
-        walk_multiples(o, q, span, wl, [&](u32 j, int) { atomicAdd(&cur[fr.lseg0 + (j >> sbshift)], 1u); });
-        continue;
-      }
-      for (int c = c0; c < c1; c++)
-        for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q)
-          atomicAdd(&cur[fr.lseg0 + (u32)(j >> sbshift)], 1u);
-    }
-  }
+  // walk 1: hits per segment of the group
+  fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32, u32) { atomicAdd(&cur[sg], 1u); });
   __syncthreads();
   // exclusive prefix sums of the counts (each thread scans a few consecutive segments, the
   // wavefront and workgroup totals give its start) and one bucket reservation per segment
@@ -298,36 +309,24 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
     if (threadIdx.x == blockDim.x - 1) total = acc;
   }
   __syncthreads();
-  for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
-    const u32 i = it.i0 + (pairs ? w >> 3 : w);
-    const u32 q = sp[i];
-    const u64 minv = spinv[i];
-    const int qi = wl.qi[q % 30];
-    const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
-    for (u32 r = 0; r < it.nfr; r++) {
-      const FillRun fr = fruns[it.fr0 + r];
-      const u64 nbytes = (u64)fr.nseg << sbshift;
-      const u64 span = 30ull * nbytes;
-      u64 o;
-      if (!first_offset(fr.base, q, minv, span, o)) continue;
-      auto put = [&](u32 j, int c) {
-        const u32 sg = fr.lseg0 + (j >> sbshift);
-        const u32 rec = ((j & smask) << 3) | (u32)c;
-        const u32 slot = atomicAdd(&cur[sg], 1u);
-        if (slot < FILL_STAGE) {
-          stage[slot] = (sg << 20) | rec;
-        } else {
-          const u32 pos = dloc[sg] + slot;
-          if (pos < bcap) bdata[(u64)gseg[sg] * bcap + pos] = rec;
-        }
-      };
-      if (!pairs && (u64)WALK_DIV * q > span) {
-        walk_multiples(o, q, span, wl, put);
-        continue;
+  // walk 2: each record to its stage slot, grouped by segment.  Almost always the item's
+  // records fit the stage (the host sizes items to half of it) and no per-record check is
+  // needed; otherwise the records beyond the stage go straight to their buckets.
+  if (total <= FILL_STAGE) {
+    fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32 j, u32 c) {
+      stage[atomicAdd(&cur[sg], 1u)] = (sg << 20) | ((j & smask) << 3) | c;
+    });
+  } else {
+    fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32 j, u32 c) {
+      const u32 rec = ((j & smask) << 3) | c;
+      const u32 slot = atomicAdd(&cur[sg], 1u);
+      if (slot < FILL_STAGE) {
+        stage[slot] = (sg << 20) | rec;
+      } else {
+        const u32 pos = dloc[sg] + slot;
+        if (pos < bcap) bdata[(u64)gseg[sg] * bcap + pos] = rec;
       }
-      for (int c = c0; c < c1; c++)
-        for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q) put((u32)j, c);
-    }
+    });
   }
   __syncthreads();
   const u32 nst = min(total, (u32)FILL_STAGE);
