@@ -218,12 +218,13 @@ __device__ __forceinline__ bool first_offset(u64 base, u32 q, u64 minv, u64 span
 }
 
 // Every hit of an item's primes in its fill group: hit(segment of the group, byte offset
-// within the run, class).  A lane takes one prime (all 8 classes) or, when the slice is
-// short, one (prime, class) pair; primes with few hits per run walk their multiples.
-template <class F>
+// within the run, class), or hit2 for two hits of one class at once (their LDS atomics can
+// then be in flight together).  A lane takes one prime (all 8 classes) or, when the slice
+// is short, one (prime, class) pair; primes with few hits per run walk their multiples.
+template <class F, class F2>
 __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __restrict__ fruns,
                                           const u32* __restrict__ sp, const u64* __restrict__ spinv,
-                                          const WheelLds& wl, int sbshift, bool pairs, F hit) {
+                                          const WheelLds& wl, int sbshift, bool pairs, F hit, F2 hit2) {
   const u32 nwork = pairs ? (it.i1 - it.i0) * 8 : it.i1 - it.i0;
   for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
     const u32 i = it.i0 + (pairs ? w >> 3 : w);
@@ -241,9 +242,13 @@ __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __r
         walk_multiples(o, q, span, wl, [&](u32 j, int c) { hit(fr.lseg0 + (j >> sbshift), j, (u32)c); });
         continue;
       }
-      for (int c = c0; c < c1; c++)
-        for (u64 j = class_start(o, q, qi, c, wl); j < nbytes; j += q)
-          hit(fr.lseg0 + (u32)(j >> sbshift), (u32)j, (u32)c);
+      for (int c = c0; c < c1; c++) {
+        // two hits per step: their LDS atomics are in flight together
+        u64 j = class_start(o, q, qi, c, wl);
+        for (; j + q < nbytes; j += 2ull * q)
+          hit2(fr.lseg0 + (u32)(j >> sbshift), (u32)j, fr.lseg0 + (u32)((j + q) >> sbshift), (u32)(j + q), (u32)c);
+        if (j < nbytes) hit(fr.lseg0 + (u32)(j >> sbshift), (u32)j, (u32)c);
+      }
     }
   }
 }
@@ -281,7 +286,11 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
   // few primes with many hits each: one lane per (prime, class); else one lane per prime
   const bool pairs = it.i1 - it.i0 < blockDim.x;
   // walk 1: hits per segment of the group
-  fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32, u32) { atomicAdd(&cur[sg], 1u); });
+  fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32, u32) { atomicAdd(&cur[sg], 1u); },
+            [&](u32 sa, u32, u32 sb, u32, u32) {
+              atomicAdd(&cur[sa], 1u);
+              atomicAdd(&cur[sb], 1u);
+            });
   __syncthreads();
   // exclusive prefix sums of the counts (each thread scans a few consecutive segments, the
   // wavefront and workgroup totals give its start) and one bucket reservation per segment
@@ -315,9 +324,14 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
   if (total <= FILL_STAGE) {
     fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32 j, u32 c) {
       stage[atomicAdd(&cur[sg], 1u)] = (sg << 20) | ((j & smask) << 3) | c;
+    }, [&](u32 sa, u32 ja, u32 sb, u32 jb, u32 c) {
+      const u32 ka = atomicAdd(&cur[sa], 1u);
+      const u32 kb = atomicAdd(&cur[sb], 1u);
+      stage[ka] = (sa << 20) | ((ja & smask) << 3) | c;
+      stage[kb] = (sb << 20) | ((jb & smask) << 3) | c;
     });
   } else {
-    fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32 j, u32 c) {
+    auto put = [&](u32 sg, u32 j, u32 c) {
       const u32 rec = ((j & smask) << 3) | c;
       const u32 slot = atomicAdd(&cur[sg], 1u);
       if (slot < FILL_STAGE) {
@@ -326,6 +340,10 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
         const u32 pos = dloc[sg] + slot;
         if (pos < bcap) bdata[(u64)gseg[sg] * bcap + pos] = rec;
       }
+    };
+    fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, put, [&](u32 sa, u32 ja, u32 sb, u32 jb, u32 c) {
+      put(sa, ja, c);
+      put(sb, jb, c);
     });
   }
   __syncthreads();
