@@ -99,11 +99,16 @@ def _engine_line(metric, value, unit, ms, args, config, extra=None):
     return 0
 
 
+def _prove(args):
+    from mathpub_amd import wf
+    return {"auto": wf.PROVE_AUTO, "off": wf.PROVE_OFF, "on": wf.PROVE_ON}[args.prove]
+
+
 def bench_c3(args):
     """4096 uneven ranges in one batched (HIP graph) call, per-range counts and checksums."""
     from mathpub_amd import wf
     ranges = c3_ranges()
-    eng = wf.Engine(arith=args.arith)
+    eng = wf.Engine(arith=args.arith, prove=_prove(args))
     t = time.perf_counter()
     tuned = None if args.no_tune else eng.autotune(ranges[0][0], ranges[0][1])
     print("autotune %s (%.1f s)" % (tuned, time.perf_counter() - t), file=sys.stderr, flush=True)
@@ -147,7 +152,8 @@ def bench_c3(args):
                                    "autotuned": tuned, "w_products_per_s": prods / (ms / 1e3),
                                    "plain_vs_batched_same_ranges": compare,
                                    "device_ms_per_step": sum(t["total_ms"] for t in tims) / len(tims),
-                                   "host_plan_ms_per_step": sum(t["plan_ms"] for t in tims) / len(tims)})
+                                   "host_plan_ms_per_step": sum(t["plan_ms"] for t in tims) / len(tims),
+                                   "prove": args.prove, "primes_proved_per_step": tims[-1]["proved"]})
 
 
 def bench_c4(args):
@@ -156,7 +162,7 @@ def bench_c4(args):
     from mathpub_amd import wf
     span = int(args.c4_span)
     lo = (1 << 64) - span
-    eng = wf.Engine()
+    eng = wf.Engine(prove=_prove(args))
     for i in range(args.warmup):
         eng.scan(lo, 1 << 64, psp=True)
     t0 = time.perf_counter()
@@ -169,6 +175,7 @@ def bench_c4(args):
                         "primes tested/s", ms, args,
                         {"workload": "[2^64-%d, 2^64), base 2, strong pseudoprimes to base 2 listed" % span,
                          "engine": "plain", "tests": r["tests"], "cks": "%016x" % r["cks"], "psp": r["psp"],
+                         "bpsw_pseudoprimes": r["nbpsw"],
                          "odd_numbers_per_s": span / 2 / (ms / 1e3), "sieve_ms": tm["sieve_ms"],
                          "test_ms": tm["test_ms"]})
 
@@ -187,7 +194,7 @@ def bench_published(args):
     from mathpub_amd import wf
     w = PUBLISHED[args.workload]
     lo, span = w["lo"], int(args.span)
-    eng = wf.Engine()
+    eng = wf.Engine(prove=_prove(args))
     eng.set_bases(w["bases"])
     tuned = None if args.no_tune else eng.autotune(lo, lo + span)
     for _ in range(args.warmup):
@@ -224,7 +231,7 @@ def bench_c5(args):
     from arrival to completion (admission waits included)."""
     import random
     from mathpub_amd import wf
-    eng = wf.Engine()
+    eng = wf.Engine(prove=_prove(args))
     eng.scan(10 ** 18, 10 ** 18 + 10 ** 6)          # device-resident sieving primes up to 1e9
     qs = c5_queries(args.queries)
     rng = random.Random(17)
@@ -266,7 +273,7 @@ def bench_c5(args):
                         sweep[-1]["wall_s"] * 1e3, args,
                         {"workload": "%d queries per rate, lengths logU[1e5,1e10], starts U[1e9,1e18], Poisson "
                                      "arrivals" % len(qs), "engine": "serving (continuous batching)",
-                         "sweep": sweep})
+                         "prove": args.prove, "sweep": sweep})
 
 
 def main():
@@ -288,6 +295,8 @@ def main():
     ap.add_argument("--c4-span", type=float, default=2.0 ** 34, help="c4: numbers below 2^64 per step")
     ap.add_argument("--c3-compare", type=int, default=0,
                     help="c3: also time the first N ranges one wf_scan at a time against one batched call")
+    ap.add_argument("--prove", choices=["auto", "off", "on"], default="auto",
+                    help="prove mode: short pieces sieved with small primes, survivors decided by BPSW")
     ap.add_argument("--test-wgs", type=int, default=0,
                     help="batched: test workgroups per CU (0 = engine default)")
     args = ap.parse_args()
@@ -300,7 +309,7 @@ def main():
         return 2
     from mathpub_amd import wf
     span = int(args.span)
-    eng = wf.Engine(arith=args.arith, test_wgs=args.test_wgs)
+    eng = wf.Engine(arith=args.arith, test_wgs=args.test_wgs, prove=_prove(args))
     print("engine: " + eng.info(), file=sys.stderr, flush=True)
 
     def timed(fn, label):

@@ -18,6 +18,8 @@
  *   (_base)        A001220/cuda/wieferich_cuda.cu:976-983 (check / selftest)
  *   wf_primes      primesieve_jump_to / primesieve_next_prime, A001220/wieferich.c:1138-1141
  *                  (the prime set of [lo, hi), for tests and the sieve benchmark)
+ *   wf_bpsw        d_sprp2_64 + d_slprp64 of A350826/cuda/a350826_cuda.cu:768-898 (its
+ *                  test_kernel / `num` command, :2458-2483)
  *   wf_serve_*     (no counterpart: a device-resident request loop)
  *
  * Conventions (as wieferich.c): return 0 on success or a negative WF_E_* code
@@ -68,6 +70,8 @@ typedef struct {
     uint64_t sumq[WF_MAXB];
     uint64_t nev;           /* events of this range (all of them, even beyond ev_cap) */
     uint64_t npsp;          /* odd composites that are base-2 strong probable primes */
+    uint64_t nbpsw;         /* of those, the ones that also pass the strong Lucas test (BPSW
+                               pseudoprimes: none exist below 2^64) */
 } wf_result;
 
 typedef struct { uint64_t lo, hi; } wf_range;   /* [lo, hi), hi == 0 means 2^64 */
@@ -82,7 +86,14 @@ typedef struct {
     uint64_t chunk;         /* numbers per device pass of wf_scan (memory bound) */
     uint32_t test_wgs;      /* wf_scan_batch: test workgroups (4 wavefronts) per CU, 1 .. 16;
                                fewer leave room for the next pass's sieve beside the tests */
+    uint32_t prove;         /* partial sieve + proof for short pieces: WF_PROVE_AUTO (by a cost
+                               model), WF_PROVE_OFF, WF_PROVE_ON (every piece above bucket_min^2) */
 } wf_config;
+
+/* Prove mode: a piece is sieved by the primes below bucket_min only, and its survivors are
+ * decided by BPSW (base-2 strong test + strong Lucas test, exact below 2^64) instead of
+ * placing every sieving prime up to sqrt(hi) in it.  Results are identical either way. */
+enum { WF_PROVE_AUTO = 0, WF_PROVE_OFF = 1, WF_PROVE_ON = 2 };
 
 /* Engine life cycle (the reference's gpu_init, wieferich_cuda.cu:1570-1577). */
 int  wf_open(int device, const wf_config *cfg, wf_engine **out);
@@ -124,6 +135,11 @@ int  wf_quotients_base(wf_engine *e, uint32_t b, const uint64_t *p, uint64_t *q,
 int  wf_primes(wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *out, uint64_t cap,
                uint64_t *count);
 
+/* BPSW verdicts of n[0..count): flags[i] bit 0 = base-2 strong probable prime, bit 1 =
+ * strong Lucas probable prime with Selfridge's parameters (both for odd n >= 3), bit 2 =
+ * prime (exact below 2^64).  Any n is accepted (even n and n < 3 give 0, or 4 for n = 2). */
+int  wf_bpsw(wf_engine *e, const uint64_t *n, uint8_t *flags, uint64_t count);
+
 /* Serving: a device-resident request loop with continuous batching.  submit admits a
  * range when a slot and the HBM budget allow (else WF_E_BUSY); poll returns its result
  * once done (*done = 1).  Results equal wf_scan of the same range. */
@@ -139,9 +155,11 @@ int  wf_serve_stop(wf_engine *e);
 int  wf_autotune(wf_engine *e, uint64_t lo, uint64_t hi, wf_config *best, double *best_ms);
 
 /* Timings of the last wf_scan / wf_scan_batch: device milliseconds of the sieve and
- * the test kernels (HIP events on the engine's stream), primes tested, device passes, and
- * the host milliseconds spent planning the passes (segments and fill work items). */
-typedef struct { double sieve_ms, test_ms, total_ms; uint64_t tested; uint64_t launches; double plan_ms; } wf_timing;
+ * the test kernels (HIP events on the engine's stream), primes tested, device passes, the
+ * host milliseconds spent planning the passes (segments and fill work items), and the
+ * primes that came through prove mode. */
+typedef struct { double sieve_ms, test_ms, total_ms; uint64_t tested; uint64_t launches; double plan_ms;
+                 uint64_t proved;   /* primes found by the prove-mode path */ } wf_timing;
 int  wf_last_timing(wf_engine *e, wf_timing *t);
 
 #ifdef __cplusplus

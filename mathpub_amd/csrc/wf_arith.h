@@ -579,21 +579,41 @@ namespace wf {
 // ---------------------------------------------------------------------------------
 // base-2 strong probable-prime test of an odd n >= 3, n < 2^64
 // ---------------------------------------------------------------------------------
-// Montgomery arithmetic mod n with R = 2^64 on 64-bit words (each product is built from
-// 32-bit multiplies).  n may exceed 2^63, so REDC keeps its carry bit and every result is
-// reduced to [0, n).
+// Montgomery arithmetic mod n with R = 2^64 in two 32-bit limbs, interleaving the rows of
+// a b with the REDC rows (each step one v_mad_u64_u32: 32 x 32 + 64 bits): 8 multiply-adds
+// and 2 low multiplies per product.  n may exceed 2^63, so the result keeps its carry bit
+// and is reduced to [0, n).
 
-// (a b + m n) / 2^64 mod n for a, b < n, ninv = -n^-1 mod 2^64
+// a b / 2^64 mod n for a, b < n, ninv = -n^-1 mod 2^64 (its low word is -n^-1 mod 2^32)
 __device__ __forceinline__ u64 mont64_mul(u64 a, u64 b, u64 n, u64 ninv) {
-  const u64 lo = a * b, hi = __umul64hi(a, b);
-  const u64 m = lo * ninv;
-  const u64 mh = __umul64hi(m, n);
-  // lo + m n has low word 0 and carries 1 unless lo == 0
-  const u64 t = hi + mh;
-  const bool c1 = t < hi;
-  const u64 r = t + (lo != 0);
-  const bool c2 = r < t;
-  return (c1 || c2 || r >= n) ? r - n : r;
+  const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+  const u32 n0 = (u32)n, n1 = (u32)(n >> 32), ni = (u32)ninv;
+  // row 0: t = a0 b
+  u64 c = (u64)a0 * b0;
+  u32 t0 = (u32)c;
+  c = (u64)a0 * b1 + (c >> 32);
+  u32 t1 = (u32)c, t2 = (u32)(c >> 32);
+  // clear word 0: t = (t + m n) / 2^32
+  u32 m = t0 * ni;
+  c = (u64)m * n0 + t0;
+  c = (u64)m * n1 + t1 + (c >> 32);
+  t0 = (u32)c;
+  c = (u64)t2 + (c >> 32);
+  t1 = (u32)c;
+  t2 = (u32)(c >> 32);
+  // row 1: t += a1 b
+  c = (u64)a1 * b0 + t0;
+  t0 = (u32)c;
+  c = (u64)a1 * b1 + t1 + (c >> 32);
+  t1 = (u32)c;
+  const u64 top = (u64)t2 + (c >> 32);
+  // clear word 0 again; r = (r1 2^32 + r0) < 2n < 2^65
+  m = t0 * ni;
+  c = (u64)m * n0 + t0;
+  c = (u64)m * n1 + t1 + (c >> 32);
+  const u64 r1 = top + (c >> 32);
+  const u64 r = (r1 << 32) | (u32)c;
+  return ((r1 >> 32) || r >= n) ? r - n : r;
 }
 
 __device__ __forceinline__ u64 mont64_dbl(u64 x, u64 n) {
@@ -663,6 +683,135 @@ __device__ __forceinline__ bool sprp2_64(u64 n) {
     if (x == mone) return true;
   }
   return false;
+}
+
+// ---------------------------------------------------------------------------------
+// Strong Lucas test (Selfridge's parameters) and BPSW, odd n < 2^64
+// ---------------------------------------------------------------------------------
+//
+// Baillie-Wagstaff (Math. Comp. 35, 1980): P = 1, Q = (1 - D)/4 with D the first of
+// 5, -7, 9, -11, ... whose Jacobi symbol (D/n) is -1.  With n + 1 = d 2^s, n is a strong
+// Lucas probable prime when U_d = 0 or V_(d 2^r) = 0 (mod n) for some 0 <= r < s.  BPSW is
+// this test after the base-2 strong test; no composite below 2^64 passes both (checked
+// against the complete list of base-2 Fermat pseudoprimes below 2^64, Feitsma and
+// Gilchrist), so below 2^64 it decides primality.  The reference runs the same pair in
+// A350826/cuda/a350826_cuda.cu:768-794 (base 2) and :855-898 (Lucas).
+//
+// Here only V is carried: the ladder keeps V_k, V_(k+1) and Q^k (the same four products
+// for a 0 and a 1 bit, so the lanes of a wavefront never diverge), and U_d = 0 is read
+// from D U_d = 2 V_(d+1) - V_d (gcd(D, n) = 1 since (D/n) = -1).
+
+__device__ __forceinline__ u64 mod_add(u64 a, u64 b, u64 n) {   // a, b < n
+  const u64 s = a + b;
+  return (s < a || s >= n) ? s - n : s;
+}
+__device__ __forceinline__ u64 mod_sub(u64 a, u64 b, u64 n) { return a >= b ? a - b : a - b + n; }
+
+// Jacobi symbol (a/m), m odd, both below 2^32
+__device__ __forceinline__ int jacobi32(u32 a, u32 m) {
+  int t = 1;
+  a %= m;
+  while (a) {
+    while (!(a & 1)) {
+      a >>= 1;
+      const u32 r = m & 7;
+      if (r == 3 || r == 5) t = -t;
+    }
+    const u32 x = a;
+    a = m;
+    m = x;
+    if ((a & 3) == 3 && (m & 3) == 3) t = -t;
+    a %= m;
+  }
+  return m == 1 ? t : 0;
+}
+
+// (D/n) for odd |D| < 2^31 and odd n: reciprocity brings it to (n mod |D| / |D|)
+__device__ __forceinline__ int jacobi_small(int D, u64 n) {
+  const u32 a = (u32)(D < 0 ? -D : D);
+  int t = (D < 0 && (n & 3) == 3) ? -1 : 1;           // (-1/n)
+  if ((a & 3) == 3 && (n & 3) == 3) t = -t;            // (a/n) = (n/a) unless both are 3 mod 4
+  const u32 hi = (u32)(n >> 32), lo = (u32)n;          // n mod a from 32-bit pieces
+  const u32 r = (u32)((((u64)(hi % a) << 32) | lo) % a);
+  return t * jacobi32(r, a);
+}
+
+__device__ __forceinline__ bool is_square64(u64 n) {
+  u64 r = (u64)__builtin_sqrt((double)n);
+  if (r > 0xFFFFFFFFull) r = 0xFFFFFFFFull;
+  while (r * r > n) r--;
+  while (r < 0xFFFFFFFFull && (r + 1) * (r + 1) <= n) r++;
+  return r * r == n;
+}
+
+// v * x mod n for a small v >= 0 (double and add), x < n
+__device__ __forceinline__ u64 mul_small_mod(u32 v, u64 x, u64 n) {
+  u64 r = 0;
+  for (int b = 31 - __builtin_clz(v | 1); b >= 0; b--) {
+    r = mod_add(r, r, n);
+    if ((v >> b) & 1) r = mod_add(r, x, n);
+  }
+  return r;
+}
+
+// Montgomery form R mod n, R = 2^64
+__device__ __forceinline__ u64 mont64_one(u64 n) {
+  const int nb = 64 - __builtin_clzll(n);
+  u64 one = 1ull << (nb - 1);
+  for (int k = nb - 1; k < 64; k++) one = mont64_dbl(one, n);
+  return one;
+}
+
+// true iff n is a strong Lucas probable prime with Selfridge's parameters (odd n >= 3)
+__device__ __forceinline__ bool slprp_64(u64 n) {
+  if (is_square64(n)) return false;                    // no D would have (D/n) = -1
+  // For a non-square n the search ends within a few hundred steps; the bound only keeps
+  // the loop finite by construction.
+  int D = 5;
+  for (;;) {
+    const int j = jacobi_small(D, n);
+    if (j < 0) break;
+    if (j == 0 && (u64)(D < 0 ? -D : D) != n) return false;   // D shares a factor with n
+    D = D > 0 ? -(D + 2) : 2 - D;
+    if (D > 1000001 || D < -1000001) return false;
+  }
+  const int Q = (1 - D) / 4;                            // exact: D = 1 (mod 4)
+  const u64 ninv = 0 - inv64(n);
+  const u64 one = mont64_one(n);
+  const u64 qa = mul_small_mod((u32)(Q < 0 ? -Q : Q), one, n);
+  const u64 Qm = Q < 0 ? mod_sub(0, qa, n) : qa;        // Q in Montgomery form
+  // n + 1 does not wrap: 2^64 - 1 is a multiple of 5 and was rejected by the search above
+  const int s = __builtin_ctzll(n + 1);
+  const u64 d = (n + 1) >> s;
+  // k = 1: V_1 = P = 1, V_2 = P^2 - 2Q, Q^1
+  u64 vk = one, vk1 = mod_sub(one, mod_add(Qm, Qm, n), n), qk = Qm;
+  for (int i = 62 - __builtin_clzll(d); i >= 0; i--) {
+    const bool bit = (d >> i) & 1;
+    const u64 t = mod_sub(mont64_mul(vk, vk1, n, ninv), qk, n);     // V_(2k+1) = V_k V_(k+1) - P Q^k
+    const u64 qq1 = mont64_mul(qk, Qm, n, ninv);
+    const u64 x = bit ? vk1 : vk;
+    const u64 qq = bit ? qq1 : qk;                                   // Q^(k + bit)
+    const u64 sq = mod_sub(mont64_mul(x, x, n, ninv), mod_add(qq, qq, n), n);   // V_(2(k + bit))
+    qk = mont64_mul(qq, qk, n, ninv);                                // Q^(2k + bit)
+    vk = bit ? t : sq;
+    vk1 = bit ? sq : t;
+  }
+  // U_d = 0  <=>  2 V_(d+1) = V_d (mod n)
+  if (mod_add(vk1, vk1, n) == vk || vk == 0) return true;
+  for (int r = 1; r < s; r++) {
+    vk = mod_sub(mont64_mul(vk, vk, n, ninv), mod_add(qk, qk, n), n);   // V_(2k) = V_k^2 - 2 Q^k
+    if (vk == 0) return true;
+    qk = mont64_mul(qk, qk, n, ninv);
+  }
+  return false;
+}
+
+// BPSW verdict bits: 1 = base-2 strong probable prime, 2 = strong Lucas probable prime,
+// 4 = prime (both, for odd n >= 3; n = 2 is prime; other even n and n < 2 are not)
+__device__ __forceinline__ u32 bpsw_flags(u64 n) {
+  if (n < 3 || !(n & 1)) return n == 2 ? 4u : 0u;
+  const u32 f = (sprp2_64(n) ? 1u : 0u) | (slprp_64(n) ? 2u : 0u);
+  return f == 3u ? 7u : f;
 }
 
 }  // namespace wf

@@ -174,8 +174,9 @@ struct SegDesc {
   u32 rid;        // range index
   u32 lo_off;     // valid numbers are base + [lo_off, hi_off)
   u32 hi_off;
-  u32 nq;         // sieving primes with q^2 <= the segment's last number
-  u32 pad;
+  u32 nq;         // sieving primes used: q^2 <= the segment's last number (and q < bucket_min
+                  // in prove mode)
+  u32 prove;      // 1: sieved by the primes below bucket_min only; k_prove decides the survivors
 };
 
 // consecutive segments seg0 .. seg0 + nseg - 1 of one range piece, base = seg0's base
@@ -191,7 +192,7 @@ struct FillRun { u64 base; u32 seg0, nseg, lseg0, pad; };
 struct FillDesc { u32 fr0, nfr, i0, i1, nseg, pad; };
 
 struct DevAcc {     // per range; every field is a 64-bit sum (checksums are per range and base)
-  unsigned long long primes, tests, near, flt, nev, npsp;
+  unsigned long long primes, tests, near, flt, nev, npsp, nbpsw;
 };
 
 struct DevEvent { u32 type, rid, j, pad; u64 p, q, r_hi, r_lo; };
@@ -511,9 +512,118 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
     if (threadIdx.x == 0) {
       cnt[s] = min(total, pcap);
       if (total > pcap) atomicOr(flags, 2u);
-      if (total) atomicAdd(&acc[d.rid].primes, (unsigned long long)total);
+      if (total && !d.prove) atomicAdd(&acc[d.rid].primes, (unsigned long long)total);
     }
     __syncthreads();
+  }
+}
+
+// Ordered compaction over a 256-thread workgroup: the values x of the lanes with keep set go
+// to dst[0..total) in thread order.  Every thread calls it; wtot holds the 4 wavefront counts.
+__device__ __forceinline__ u32 block_compact(bool keep, u32 x, u32* __restrict__ dst, u32* wtot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const u64 m = __ballot(keep);
+  if (lane == 0) wtot[wave] = (u32)__popcll(m);
+  __syncthreads();
+  u32 before = 0, total = 0;
+  for (int w = 0; w < 4; w++) {
+    const u32 c = wtot[w];
+    if (w < wave) before += c;
+    total += c;
+  }
+  if (keep) dst[before + (u32)__popcll(m & ((1ull << lane) - 1))] = x;
+  __syncthreads();   // wtot and dst are read again by the caller's next step
+  return total;
+}
+
+// Prove mode.  The segment was sieved by the primes below bucket_min only, so its list holds
+// every prime and the composites whose prime factors all exceed that bound.  A survivor below
+// q_next^2 (q_next = the first sieving prime left out) is prime; any other is decided by
+// BPSW, which has no pseudoprime below 2^64.
+//
+// k_prove: a work item is one tile of PROVE_TILE survivors of one segment (items run on
+// every CU, whatever the number of segments).  Phase 1 runs the base-2 strong test and keeps
+// the passes in LDS, in order; phase 2 runs the strong Lucas test on them with every lane
+// busy and writes the primes to the front of the tile, in order; tcnt holds the tile's count.
+// k_prove_gather then closes the gaps: per segment, tile after tile, it moves each tile's
+// primes down to the running total (each move stays below the next tile's start).
+#define PROVE_TILE 1024
+
+__global__ void __launch_bounds__(256) k_prove(const SegDesc* __restrict__ segs, u32 nseg, u32 ptps,
+                                               const u32* __restrict__ sp, u32* __restrict__ off,
+                                               const u32* __restrict__ cnt, u32 pcap, u32* __restrict__ tcnt) {
+  __shared__ u32 queue[PROVE_TILE];
+  __shared__ u32 wtot[4];
+  const u64 nitems = (u64)nseg * ptps;
+  for (u64 t = blockIdx.x; t < nitems; t += gridDim.x) {
+    const u32 s = (u32)(t / ptps);
+    const u32 t0 = (u32)(t - (u64)s * ptps) * PROVE_TILE;
+    const SegDesc d = segs[s];
+    if (!d.prove) continue;                         // uniform across the workgroup
+    const u32 nin = cnt[s];
+    if (t0 >= nin) continue;
+    const u32 t1 = min(nin, t0 + PROVE_TILE);
+    u32* o = off + (u64)s * pcap;
+    const u64 qn = sp[d.nq];
+    const u64 lim = qn * qn;                        // q_next < 2^32
+    u32 nq = 0;
+    for (u32 j0 = t0; j0 < t1; j0 += 256) {
+      const u32 i = j0 + threadIdx.x;
+      u32 x = 0;
+      bool keep = false;
+      if (i < t1) {
+        x = o[i];
+        const u64 n = d.base + x;
+        keep = n < lim || sprp2_64(n);
+      }
+      nq += block_compact(keep, x, queue + nq, wtot);
+    }
+    u32 wp = 0;
+    for (u32 k0 = 0; k0 < nq; k0 += 256) {
+      const u32 k = k0 + threadIdx.x;
+      u32 x = 0;
+      bool keep = false;
+      if (k < nq) {
+        x = queue[k];
+        const u64 n = d.base + x;
+        keep = n < lim || slprp_64(n);
+      }
+      wp += block_compact(keep, x, o + t0 + wp, wtot);
+    }
+    if (threadIdx.x == 0) tcnt[t] = wp;
+    __syncthreads();   // the queue is refilled by the next item
+  }
+}
+
+__global__ void __launch_bounds__(PROVE_TILE) k_prove_gather(const SegDesc* __restrict__ segs, u32 nseg, u32 ptps,
+                                                             u32* __restrict__ off, u32* __restrict__ cnt, u32 pcap,
+                                                             const u32* __restrict__ tcnt, DevAcc* __restrict__ acc,
+                                                             unsigned long long* __restrict__ proved) {
+  for (u32 s = blockIdx.x; s < nseg; s += gridDim.x) {
+    const SegDesc d = segs[s];
+    if (!d.prove) continue;
+    const u32 ntile = (cnt[s] + PROVE_TILE - 1) / PROVE_TILE;
+    u32* o = off + (u64)s * pcap;
+    u32 pos = 0;
+    for (u32 k = 0; k < ntile; k++) {
+      const u32 c = tcnt[(u64)s * ptps + k];
+      const u32 src = k * PROVE_TILE;
+      if (src != pos) {                             // uniform
+        const u32 x = threadIdx.x < c ? o[src + threadIdx.x] : 0;
+        __syncthreads();                            // every read before any write of this move
+        if (threadIdx.x < c) o[pos + threadIdx.x] = x;
+        __syncthreads();
+      }
+      pos += c;
+    }
+    if (threadIdx.x == 0) {
+      cnt[s] = pos;
+      if (pos) {
+        atomicAdd(&acc[d.rid].primes, (unsigned long long)pos);
+        atomicAdd(proved, (unsigned long long)pos);
+      }
+    }
+    __syncthreads();   // cnt[s] is read above by every thread of the workgroup
   }
 }
 
@@ -824,10 +934,18 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
         const unsigned long long kk = atomicAdd(count, 1ull);
         if (kk < cap) out[kk] = n;
         atomicAdd(&acc[d.rid].npsp, 1ull);
+        if (slprp_64(n)) atomicAdd(&acc[d.rid].nbpsw, 1ull);   // a BPSW pseudoprime: none below 2^64
       }
     }
     __syncthreads();   // the queue is refilled by the next tile
   }
+}
+
+// BPSW verdicts of arbitrary n (bits: 1 base-2 strong probable prime, 2 strong Lucas
+// probable prime, 4 prime).
+__global__ void k_bpsw(const u64* __restrict__ n_in, unsigned char* __restrict__ out, u64 n) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (unsigned char)bpsw_flags(n_in[i]);
 }
 
 // Zero n words (a kernel, so that the batched path can record it into a HIP graph).
@@ -852,6 +970,8 @@ struct BufSet {   // bucket counters / records, prime offsets / counts of one pa
   u64 cnt_cap = 0;
   u32* bitmap = nullptr;   // composite bitmap (strong-pseudoprime pass only)
   u64 bitmap_cap = 0;
+  u32* tcnt = nullptr;     // primes per prove tile
+  u64 tcnt_cap = 0;
 };
 
 struct wf_engine {
@@ -875,6 +995,7 @@ struct wf_engine {
   u32* d_sp = nullptr;
   u64* d_spinv = nullptr;
   u32 tierA = 0, q1idx = 0;
+  double surv_frac = 1.0;        // share of numbers with no prime factor below bucket_min
   // pass descriptors (all passes of a batch) and two sets of pass buffers
   SegDesc* d_segs = nullptr;
   u64 seg_cap = 0;
@@ -973,6 +1094,17 @@ static void host_primes(u64 lim, std::vector<u32>& out) {
   for (auto& v : part) out.insert(out.end(), v.begin(), v.end());
 }
 
+// Primes >= cfg.bucket_min go through k_bucket_fill (index q1idx on); the numbers free of
+// every smaller prime (the survivors of prove mode) are the share
+// (8/30) prod_{7 <= q < bucket_min} (1 - 1/q) of all numbers.
+static void set_bucket_min(wf_engine* e) {
+  e->q1idx = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), e->cfg.bucket_min) - e->h_sp.begin());
+  double f = 8.0 / 30.0;
+  for (u32 i = 0; i < e->q1idx; i++) f *= 1.0 - 1.0 / e->h_sp[i];
+  e->surv_frac = f;
+  e->slices.clear();
+}
+
 static int ensure_sieving_primes(wf_engine* e, u64 lim) {
   // primes below 2^32 - 1 sieve every range below 2^64 (the table limit is capped there,
   // so asking for more must not rebuild it)
@@ -995,11 +1127,9 @@ static int ensure_sieving_primes(wf_engine* e, u64 lim) {
   e->sp_lim = want;
   // tier A: primes below 64 * 8 = 512 are striped over a wavefront
   e->tierA = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), 512u) - e->h_sp.begin());
-  // primes >= the bucket threshold go through k_bucket_fill
-  e->q1idx = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), e->cfg.bucket_min) - e->h_sp.begin());
   e->h_inv_sum.assign(e->h_sp.size() + 1, 0.0);
   for (size_t i = 0; i < e->h_sp.size(); i++) e->h_inv_sum[i + 1] = e->h_inv_sum[i] + 1.0 / e->h_sp[i];
-  e->slices.clear();
+  set_bucket_min(e);
   return WF_OK;
 }
 
@@ -1039,7 +1169,7 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
   if (e->cfg.test_wgs > 16 || e->cfg.seg_bytes < 4096 || (e->cfg.seg_bytes & (e->cfg.seg_bytes - 1)) ||
       e->cfg.seg_bytes > SEG_MAX || e->cfg.seg_bytes + 4096 > e->prop.sharedMemPerBlock || e->cfg.sieve_block < 64 ||
       e->cfg.sieve_block > 1024 || (e->cfg.sieve_block & 63) || e->cfg.test_block != 256 || e->cfg.arith > 1 ||
-      e->cfg.bucket_min < 512 || e->cfg.chunk < 30ull * e->cfg.seg_bytes) {
+      e->cfg.bucket_min < 512 || e->cfg.chunk < 30ull * e->cfg.seg_bytes || e->cfg.prove > WF_PROVE_ON) {
     delete e;
     return WF_E_ARG;
   }
@@ -1136,6 +1266,7 @@ extern "C" void wf_close(wf_engine* e) {
     (void)hipFree(e->bs[k].off);
     (void)hipFree(e->bs[k].cnt);
     (void)hipFree(e->bs[k].bitmap);
+    (void)hipFree(e->bs[k].tcnt);
   }
   (void)hipFree(e->d_psp);
   (void)hipFree(e->d_pre);
@@ -1241,7 +1372,7 @@ static double span_d(u64 lo, u64 hi) { return hi ? (double)(hi - lo) : 184467440
 static u64 prime_bound(double y) { return y < 100 ? 64 : (u64)(2.0 * y / std::log(y)) + 64; }
 
 // Segment descriptors of [lo, hi) and the runs (<= RUN_MAX consecutive segments) they form.
-static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vector<SegDesc>& segs,
+static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, bool prove, std::vector<SegDesc>& segs,
                            std::vector<RunDesc>& runs) {
   const u32 SB = e->cfg.seg_bytes;
   const u64 span = 30ull * SB;
@@ -1263,7 +1394,11 @@ static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vec
     const u64 r = isqrt_u64(last);
     d.nq = (u32)(std::upper_bound(e->h_sp.begin(), e->h_sp.end(), (u32)std::min<u64>(r, 0xFFFFFFFFull)) -
                  e->h_sp.begin());
-    d.pad = 0;
+    d.prove = 0;
+    if (prove && d.nq > e->q1idx) {   // sieve with the primes below bucket_min, prove the rest
+      d.nq = e->q1idx;
+      d.prove = 1;
+    }
     if (runs.empty() || runs.back().seg0 + runs.back().nseg != segs.size() || runs.back().nseg == RUN_MAX ||
         segs.empty() || segs.back().rid != rid || segs.back().base + span != base) {
       RunDesc rd;
@@ -1289,8 +1424,40 @@ struct PassPlan {
   std::vector<RunDesc> runs;
   std::vector<FillRun> fruns;
   std::vector<FillDesc> fills;
-  u32 maxnq = 0, bcap = 0, pcap = 0, tps = 0;
+  u32 maxnq = 0, bcap = 0, pcap = 0, tps = 0, nprove = 0;
 };
+
+// Prove-mode cost model, in nanoseconds of the whole device, measured on MI355X: placing
+// one sieving prime in one piece (k_bucket_fill: 8 ps at 1e17 .. 2^64), and BPSW per
+// survivor (base-2 strong test, plus the strong Lucas test for the half that are prime:
+// 0.26 ns), doubled because the bucket fill overlaps the tests of the batched path and
+// BPSW, all VALU work, does not.
+#define PROVE_T_PLACE 0.008
+#define PROVE_T_SURV 0.52
+
+// Whether piece [lo, hi) should run in prove mode: its sieving primes from bucket_min up
+// would each be placed at least once, against BPSW for its survivors.
+static bool prove_piece(const wf_engine* e, u64 lo, u64 hi) {
+  if (e->cfg.prove == WF_PROVE_OFF) return false;
+  const u64 top = hi ? hi - 1 : 0xFFFFFFFFFFFFFFFFull;
+  const u32 nq = (u32)(std::upper_bound(e->h_sp.begin(), e->h_sp.end(),
+                                        (u32)std::min<u64>(isqrt_u64(top), 0xFFFFFFFFull)) - e->h_sp.begin());
+  if (nq <= e->q1idx) return false;   // nothing to leave out
+  if (e->cfg.prove == WF_PROVE_ON) return true;
+  return span_d(lo, hi) * e->surv_frac * PROVE_T_SURV < (double)(nq - e->q1idx) * PROVE_T_PLACE;
+}
+
+// Entries per segment list: the primes of a segment, or in prove mode its survivors (their
+// expected count E = surv_frac numbers, with room for the spread of a sieved count).
+static u32 list_cap(const wf_engine* e, bool any_prove) {
+  const double y = 30.0 * e->cfg.seg_bytes;
+  u64 cap = prime_bound(y);
+  if (any_prove) {
+    const double E = y * e->surv_frac;
+    cap = std::max<u64>(cap, (u64)(1.25 * E + 8.0 * std::sqrt(E) + 512.0));
+  }
+  return (u32)((cap + 63) & ~63ull);
+}
 
 // Slice ends for the primes [i, gend) of one [Q, 2Q) group over runs of ns segments: each
 // slice holds about FILL_TARGET expected records (prime q leaves 8 seg_bytes ns / q of them),
@@ -1314,14 +1481,17 @@ static const std::vector<u32>& fill_slices(wf_engine* e, u32 i, u32 gend, u32 ns
   return e->slices.emplace(key, std::move(ends)).first->second;
 }
 
-static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& P) {
+static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& P, bool allow_prove = true) {
   P.segs.clear();
   P.runs.clear();
   P.fruns.clear();
-  for (const Piece& p : pieces) build_segments(e, p.lo, p.hi, p.rid, P.segs, P.runs);
+  for (const Piece& p : pieces)
+    build_segments(e, p.lo, p.hi, p.rid, allow_prove && prove_piece(e, p.lo, p.hi), P.segs, P.runs);
   P.maxnq = 0;
   for (const RunDesc& r : P.runs) P.maxnq = std::max(P.maxnq, r.nq);
-  P.pcap = (u32)prime_bound(30.0 * e->cfg.seg_bytes);
+  P.nprove = 0;
+  for (const SegDesc& d : P.segs) P.nprove += d.prove;
+  P.pcap = list_cap(e, P.nprove > 0);
   P.tps = (P.pcap + TEST_TILE - 1) / TEST_TILE;
   P.bcap = 0;
   P.fills.clear();
@@ -1391,6 +1561,9 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
   }
 }
 
+// Prove tiles per segment list.
+static u32 prove_tiles(const PassPlan& P) { return (P.pcap + PROVE_TILE - 1) / PROVE_TILE; }
+
 // Make a buffer set large enough for pass P.
 static int size_set(BufSet& B, const PassPlan& P) {
   const u64 nseg = P.segs.size();
@@ -1401,6 +1574,7 @@ static int size_set(BufSet& B, const PassPlan& P) {
     if ((rc = grow(B.bcnt, B.bcnt_cap, nseg)) != WF_OK) return rc;
     if ((rc = grow(B.bdata, B.bdata_cap, nseg * (u64)P.bcap)) != WF_OK) return rc;
   }
+  if (P.nprove && (rc = grow(B.tcnt, B.tcnt_cap, nseg * (u64)prove_tiles(P))) != WF_OK) return rc;
   return WF_OK;
 }
 
@@ -1422,6 +1596,15 @@ static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, 
                      e->d_spinv, e->tierA, e->q1idx, B.bcnt, B.bdata, P.bcap, B.off, B.cnt, P.pcap, acc, e->d_flags,
                      want_bitmap ? B.bitmap : nullptr, e->cfg.seg_bytes / 4, e->d_pre, e->pre_words);
   HIPCHK(hipGetLastError());
+  if (P.nprove) {
+    const u32 ptps = prove_tiles(P);
+    hipLaunchKernelGGL(k_prove, dim3((u32)std::min<u64>((u64)P.nprove * ptps, (u64)cus * 16)), dim3(256), 0, st,
+                       dsegs, (u32)nseg, ptps, e->d_sp, B.off, B.cnt, P.pcap, B.tcnt);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_prove_gather, dim3((u32)std::min<u64>(P.nprove, (u64)cus * 2)), dim3(PROVE_TILE), 0, st,
+                       dsegs, (u32)nseg, ptps, B.off, B.cnt, P.pcap, B.tcnt, acc, e->d_ctr + 3);
+    HIPCHK(hipGetLastError());
+  }
   return WF_OK;
 }
 
@@ -1471,7 +1654,7 @@ static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith,
                       bool want_psp = false) {
   PassPlan P;
   const auto t0 = std::chrono::steady_clock::now();
-  plan_pass(e, pieces, P);
+  plan_pass(e, pieces, P, !want_psp);   // the pseudoprime pass needs the full composite bitmap
   e->last.plan_ms += ms_since(t0);
   if (P.segs.empty()) return WF_OK;
   int rc;
@@ -1626,7 +1809,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   HIPCHK(hipMemsetAsync(e->d_acc, 0, n * sizeof(DevAcc), e->st));
   HIPCHK(hipMemsetAsync(e->d_cks, 0, ncks * sizeof(unsigned long long), e->st));
   HIPCHK(hipMemsetAsync(e->d_sumq, 0, ncks * sizeof(unsigned long long), e->st));
-  HIPCHK(hipMemsetAsync(e->d_ctr, 0, 3 * sizeof(unsigned long long), e->st));
+  HIPCHK(hipMemsetAsync(e->d_ctr, 0, 4 * sizeof(unsigned long long), e->st));
   if (want_psp && (rc = grow(e->d_psp, e->psp_cap, std::max<u64>(psp_cap, 1 << 16))) != WF_OK) return rc;
   HIPCHK(hipMemsetAsync(e->d_flags, 0, sizeof(u32), e->st));
   TestOut to;
@@ -1730,7 +1913,13 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   }
   e->last.tested = tested;
   if (!batched) e->last.total_ms = e->last.sieve_ms + e->last.test_ms;
-  for (u32 i = 0; i < n; i++) res[i].npsp = acc[i].npsp;
+  for (u32 i = 0; i < n; i++) {
+    res[i].npsp = acc[i].npsp;
+    res[i].nbpsw = acc[i].nbpsw;
+  }
+  unsigned long long proved = 0;
+  HIPCHK(hipMemcpy(&proved, e->d_ctr + 3, sizeof proved, hipMemcpyDeviceToHost));
+  e->last.proved = proved;
   if (want_psp) {
     unsigned long long np = 0;
     HIPCHK(hipMemcpy(&np, e->d_ctr + 2, sizeof np, hipMemcpyDeviceToHost));
@@ -1788,6 +1977,26 @@ extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* 
   return wf_quotients_base(e, 2, p, q, r_hi, r_lo, n);
 }
 
+extern "C" int wf_bpsw(wf_engine* e, const u64* n, uint8_t* flags, u64 count) {
+  if (!e || (count && (!n || !flags))) return WF_E_ARG;
+  if (count == 0) return WF_OK;
+  HIPCHK(hipSetDevice(e->dev));
+  u64* d = nullptr;
+  if (hipMalloc(&d, count * (sizeof(u64) + 1)) != hipSuccess) return WF_E_NOMEM;
+  unsigned char* dout = (unsigned char*)(d + count);
+  int rc = WF_OK;
+  if (hipMemcpyAsync(d, n, count * sizeof(u64), hipMemcpyHostToDevice, e->st) != hipSuccess) rc = WF_E_DEVICE;
+  if (rc == WF_OK) {
+    hipLaunchKernelGGL(k_bpsw, dim3((u32)((count + 255) / 256)), dim3(256), 0, e->st, d, dout, count);
+    if (hipGetLastError() != hipSuccess) rc = WF_E_DEVICE;
+  }
+  if (rc == WF_OK && (hipMemcpyAsync(flags, dout, count, hipMemcpyDeviceToHost, e->st) != hipSuccess ||
+                      hipStreamSynchronize(e->st) != hipSuccess))
+    rc = WF_E_DEVICE;
+  (void)hipFree(d);
+  return rc;
+}
+
 extern "C" int wf_primes(wf_engine* e, u64 lo, u64 hi, u64* out, u64 cap, u64* count) {
   if (!e || !count || (cap && !out) || (hi != 0 && hi <= lo)) return WF_E_ARG;
   HIPCHK(hipSetDevice(e->dev));
@@ -1838,8 +2047,7 @@ static void apply_config(wf_engine* e, const wf_config& c) {
   e->cfg.sieve_block = c.sieve_block;
   e->cfg.bucket_min = c.bucket_min;
   e->sbshift = __builtin_ctz(e->cfg.seg_bytes);
-  if (!e->h_sp.empty())
-    e->q1idx = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), e->cfg.bucket_min) - e->h_sp.begin());
+  if (!e->h_sp.empty()) set_bucket_min(e);
   e->slices.clear();
 }
 
@@ -2034,9 +2242,10 @@ extern "C" int wf_serve_start(wf_engine* e, u32 slots, u64 hbm_budget) {
   Server* S = new Server();
   S->slots = slots;
   S->slot_ticket.assign(slots, 0);
-  // bytes of pass buffers per number: prime regions + buckets (large primes up to 2^32)
+  // bytes of pass buffers per number: prime regions (survivor lists in prove mode, at most
+  // a tenth of the numbers for bucket_min >= 512) + buckets (large primes up to 2^32)
   const double SB = e->cfg.seg_bytes;
-  const double pcap = (double)prime_bound(30.0 * SB);
+  const double pcap = std::max((double)prime_bound(30.0 * SB), e->cfg.prove == WF_PROVE_OFF ? 0.0 : 3.0 * SB);
   const double bcap = 8.0 * SB * 0.70 + 10.0 * std::sqrt(8.0 * SB * 0.70) + 1024.0;
   const double per_number = (pcap + bcap) * 4.0 / (30.0 * SB);
   if (!hbm_budget) hbm_budget = 16ull << 30;

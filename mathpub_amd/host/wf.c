@@ -271,10 +271,22 @@ static void bases_str(char *buf, size_t n)
 
 /* ---------------------------------------------------------------- engine */
 
+static wf_config g_cfg;   /* zero fields: engine defaults */
+
+/* --prove auto|off|on: prove mode of the engine (wf.h, WF_PROVE_*) */
+static u32 parse_prove(const char *s)
+{
+    if (!strcmp(s, "auto")) return WF_PROVE_AUTO;
+    if (!strcmp(s, "off")) return WF_PROVE_OFF;
+    if (!strcmp(s, "on")) return WF_PROVE_ON;
+    die("--prove takes auto, off or on, not '%s'", s);
+    return 0;
+}
+
 static void engine_open(void)
 {
     if (g_eng) return;
-    int rc = wf_open(g_dev, NULL, &g_eng);
+    int rc = wf_open(g_dev, &g_cfg, &g_eng);
     if (rc) die("cannot open GPU %d: %s", g_dev, wf_strerror(rc));
     rc = wf_set_bases(g_eng, g_b, g_nb);
     if (rc) {
@@ -618,6 +630,8 @@ static void usage(void)
           "       wf check P [-b BASES] [-g DEV]\n"
           "       wf bench [N] [-d SPAN] [-b BASES] [-g DEV]\n"
           "       wf selftest [LIMIT] [-g DEV]\n"
+          "       wf num X [X ...] [-g DEV]          BPSW tests of single numbers on the GPU\n"
+          "scan --prove auto|off|on: sieve short chunks with small primes only and prove the rest by BPSW\n"
           "BASES: list/ranges like 3,5,7 or 2-30; 'oeis' (default) = 3,5,6,7,10,12,13,14,15,17,18,19,20,22,23,26,30;\n"
           "       'known' = the bases 2..30 that are not perfect powers (default of check)\n"
           "END may be 2^64\n", stderr);
@@ -646,6 +660,7 @@ static int cmd_scan(int argc, char **argv)
         else if (!strcmp(a, "-g") && i + 1 < argc) g_dev = parse_int(argv[++i], 0, 64, "device");
         else if (!strcmp(a, "-q")) quiet = true;
         else if (!strcmp(a, "-Q")) sumq = true;
+        else if (!strcmp(a, "--prove") && i + 1 < argc) g_cfg.prove = parse_prove(argv[++i]);
         else if (a[0] != '-' && npos < 2) pos[npos++] = parse_num128(a);
         else usage();
     }
@@ -1024,6 +1039,30 @@ static int cmd_selftest(int argc, char **argv)
     return bad ? 1 : 0;
 }
 
+/* num X [X ...]: the GPU's base-2 strong test, strong Lucas test and BPSW verdict of each X
+ * (the reference's `a350826_cuda num`, A350826/cuda/a350826_cuda.cu:2458-2483). */
+static int cmd_num(int argc, char **argv)
+{
+    u64 xs[256];
+    int n = 0;
+    for (int i = 0; i < argc; i++) {
+        if (!strcmp(argv[i], "-g") && i + 1 < argc) g_dev = parse_int(argv[++i], 0, 64, "device");
+        else if (argv[i][0] != '-' && n < 256) xs[n++] = parse_num(argv[i]);
+        else usage();
+    }
+    if (!n) usage();
+    g_b[0] = 2;
+    g_nb = 1;
+    engine_open();
+    uint8_t f[256];
+    const int rc = wf_bpsw(g_eng, xs, f, (u64)n);
+    if (rc) die("wf_bpsw: %s", wf_strerror(rc));
+    for (int i = 0; i < n; i++)
+        printf("%" PRIu64 ": gpu sprp2 %d slprp %d prime %d\n", xs[i], f[i] & 1, (f[i] >> 1) & 1, (f[i] >> 2) & 1);
+    wf_close(g_eng);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     stderr_tty = isatty(2);
@@ -1033,6 +1072,7 @@ int main(int argc, char **argv)
     if (!strcmp(cmd, "check")) return cmd_check(argc - 2, argv + 2);
     if (!strcmp(cmd, "bench")) return cmd_bench(argc - 2, argv + 2);
     if (!strcmp(cmd, "selftest")) return cmd_selftest(argc - 2, argv + 2);
+    if (!strcmp(cmd, "num")) return cmd_num(argc - 2, argv + 2);
     usage();
     return 2;
 }

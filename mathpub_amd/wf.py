@@ -11,6 +11,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(PKG, "libwf.so")
 MAXB = 64
 EV_FOUND, EV_NEAR, EV_FLT = 0, 1, 2
+PROVE_AUTO, PROVE_OFF, PROVE_ON = 0, 1, 2
+BPSW_SPRP2, BPSW_LUCAS, BPSW_PRIME = 1, 2, 4
 
 u64, u32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64
 
@@ -21,7 +23,7 @@ class Event(ctypes.Structure):
 
 class Result(ctypes.Structure):
     _fields_ = [("primes", u64), ("tests", u64), ("near", u64), ("flt", u64), ("cks", u64 * MAXB),
-                ("sumq", u64 * MAXB), ("nev", u64), ("npsp", u64)]
+                ("sumq", u64 * MAXB), ("nev", u64), ("npsp", u64), ("nbpsw", u64)]
 
 
 class Range(ctypes.Structure):
@@ -30,12 +32,12 @@ class Range(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("seg_bytes", u32), ("sieve_block", u32), ("test_block", u32), ("arith", u32), ("bucket_min", u32),
-                ("chunk", u64), ("test_wgs", u32)]
+                ("chunk", u64), ("test_wgs", u32), ("prove", u32)]
 
 
 class Timing(ctypes.Structure):
     _fields_ = [("sieve_ms", ctypes.c_double), ("test_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("tested", u64), ("launches", u64), ("plan_ms", ctypes.c_double)]
+                ("tested", u64), ("launches", u64), ("plan_ms", ctypes.c_double), ("proved", u64)]
 
 
 class WFError(RuntimeError):
@@ -70,6 +72,7 @@ def load():
     L.wf_quotients.argtypes = [ctypes.c_void_p, P64, P64, P64, P64, u64]
     L.wf_quotients_base.argtypes = [ctypes.c_void_p, u32, P64, P64, P64, P64, u64]
     L.wf_primes.argtypes = [ctypes.c_void_p, u64, u64, P64, u64, P64]
+    L.wf_bpsw.argtypes = [ctypes.c_void_p, P64, ctypes.POINTER(ctypes.c_uint8), u64]
     L.wf_last_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(Timing)]
     L.wf_autotune.argtypes = [ctypes.c_void_p, u64, u64, ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_double)]
     L.wf_serve_start.argtypes = [ctypes.c_void_p, u32, u64]
@@ -77,7 +80,7 @@ def load():
     L.wf_serve_poll.argtypes = [ctypes.c_void_p, u64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_int)]
     L.wf_serve_stop.argtypes = [ctypes.c_void_p]
     for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_quotients_base",
-              "wf_primes",
+              "wf_primes", "wf_bpsw",
               "wf_last_timing", "wf_autotune", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
         getattr(L, f).restype = ctypes.c_int
     _lib = L
@@ -102,15 +105,16 @@ def result_dict(res, events=None, nb=1):
     the engine's order; events as (type, p, q, r, base) sorted by (p, base)."""
     return {"primes": res.primes, "tests": res.tests, "near": res.near, "flt": res.flt,
             "cks": res.cks[0], "sumq": res.sumq[0], "cks_all": list(res.cks[:nb]), "sumq_all": list(res.sumq[:nb]),
-            "nev": res.nev, "npsp": res.npsp, "events": events if events is not None else []}
+            "nev": res.nev, "npsp": res.npsp, "nbpsw": res.nbpsw, "events": events if events is not None else []}
 
 
 class Engine:
     """One GPU engine (wf_open); not thread-safe, like the C object."""
 
-    def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0, bucket_min=0, test_wgs=0):
+    def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0, bucket_min=0, test_wgs=0,
+                 prove=PROVE_AUTO):
         L = load()
-        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs)
+        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs, prove)
         h = ctypes.c_void_p()
         _check(L.wf_open(device, ctypes.byref(cfg), ctypes.byref(h)), "wf_open")
         self._h = h
@@ -184,6 +188,14 @@ class Engine:
         _check(self._L.wf_primes(self._h, lo, _hi(hi), buf, cnt.value, ctypes.byref(cnt)), "wf_primes")
         return list(buf[:cnt.value])
 
+    def bpsw(self, ns):
+        """BPSW flags of each n (wf_bpsw): BPSW_SPRP2 | BPSW_LUCAS | BPSW_PRIME bits."""
+        n = len(ns)
+        arr = (u64 * max(n, 1))(*ns)
+        out = (ctypes.c_uint8 * max(n, 1))()
+        _check(self._L.wf_bpsw(self._h, arr, out, n), "wf_bpsw")
+        return list(out[:n])
+
     def autotune(self, lo, hi):
         """Pick segment size, sieve workgroup and bucket threshold for [lo, hi) (wf_autotune)."""
         c, ms = Config(), ctypes.c_double()
@@ -217,4 +229,4 @@ class Engine:
         t = Timing()
         _check(self._L.wf_last_timing(self._h, ctypes.byref(t)), "wf_last_timing")
         return {"sieve_ms": t.sieve_ms, "test_ms": t.test_ms, "total_ms": t.total_ms, "tested": t.tested,
-                "launches": t.launches, "plan_ms": t.plan_ms}
+                "launches": t.launches, "plan_ms": t.plan_ms, "proved": t.proved}

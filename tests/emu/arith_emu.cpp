@@ -90,6 +90,12 @@ int emu_sprp2(uint64_t n) { return sprp2_64(n) ? 1 : 0; }
 
 int emu_psp_possible(uint64_t n) { return psp_possible(n) ? 1 : 0; }
 
+int emu_slprp(uint64_t n) { return slprp_64(n) ? 1 : 0; }
+
+uint32_t emu_bpsw(uint64_t n) { return bpsw_flags(n); }
+
+int emu_jacobi(int D, uint64_t n) { return jacobi_small(D, n); }
+
 void emu_m28(uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
   uint32_t x[4];
   m28_pow2(p, p - 1, ebits, x);

@@ -213,3 +213,128 @@ def test_radix28_any_base(emu):
                 continue
             eb = min(64, (p - 1).bit_length() + rng.randint(0, 2))
             assert call(emu.emu_m28c, b, p, eb) == pow(b, p - 1, p * p), (b, p)
+
+
+# ---- strong Lucas (Selfridge) and BPSW --------------------------------------------------
+
+def _lucas_uv_matrix(P, Q, k, n):
+    """U_k, V_k mod n from the 2x2 matrix [[P, -Q], [1, 0]]^k = [[U_(k+1), -Q U_k], [U_k, -Q U_(k-1)]],
+    V_k = 2 U_(k+1) - P U_k: the definition, by a route unlike the kernel's V ladder."""
+    def mul(A, B):
+        return [[(A[0][0] * B[0][0] + A[0][1] * B[1][0]) % n, (A[0][0] * B[0][1] + A[0][1] * B[1][1]) % n],
+                [(A[1][0] * B[0][0] + A[1][1] * B[1][0]) % n, (A[1][0] * B[0][1] + A[1][1] * B[1][1]) % n]]
+    R = [[1, 0], [0, 1]]
+    M = [[P % n, (-Q) % n], [1, 0]]
+    while k:
+        if k & 1:
+            R = mul(R, M)
+        M = mul(M, M)
+        k >>= 1
+    u1, u = R[0][0], R[1][0]
+    return u, (2 * u1 - P * u) % n
+
+
+def _jacobi(a, n):
+    a %= n
+    t = 1
+    while a:
+        while a % 2 == 0:
+            a //= 2
+            if n % 8 in (3, 5):
+                t = -t
+        a, n = n, a
+        if a % 4 == 3 and n % 4 == 3:
+            t = -t
+        a %= n
+    return t if n == 1 else 0
+
+
+def _slprp_def(n):
+    r = int(n ** 0.5)
+    while r * r > n:
+        r -= 1
+    while (r + 1) ** 2 <= n:
+        r += 1
+    if r * r == n:
+        return False
+    D = 5
+    while True:
+        j = _jacobi(D, n)
+        if j == -1:
+            break
+        if j == 0 and abs(D) != n:
+            return False
+        D = -(D + 2) if D > 0 else 2 - D
+    Q = (1 - D) // 4
+    d, s = n + 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    u, v = _lucas_uv_matrix(1, Q, d, n)
+    if u == 0 or v == 0:
+        return True
+    return any(_lucas_uv_matrix(1, Q, d << r, n)[1] == 0 for r in range(1, s))
+
+
+@pytest.fixture(scope="module")
+def emu_lucas(emu):
+    emu.emu_slprp.argtypes = [ctypes.c_uint64]
+    emu.emu_bpsw.argtypes = [ctypes.c_uint64]
+    emu.emu_bpsw.restype = ctypes.c_uint32
+    emu.emu_jacobi.argtypes = [ctypes.c_int, ctypes.c_uint64]
+    return emu
+
+
+def test_jacobi_small(emu_lucas):
+    from sympy import jacobi_symbol
+    rng = random.Random(31)
+    ns = [rng.getrandbits(64) | 1 for _ in range(400)] + [3, 5, 7, 9, 15, 21, 25, 5459, (1 << 64) - 1]
+    for n in ns:
+        for D in (5, -7, 9, -11, 13, -15, 17, -19, 21, -23, 25, -101, 1001):
+            assert emu_lucas.emu_jacobi(D, n) == jacobi_symbol(D, n), (D, n)
+
+
+def test_strong_lucas_small_exhaustive(emu_lucas):
+    """Every odd n < 30000 against the matrix-power definition; the composites that pass are
+    the first strong Lucas pseudoprimes (OEIS A217255)."""
+    passing_composites = []
+    for n in range(3, 30000, 2):
+        got = bool(emu_lucas.emu_slprp(n))
+        assert got == _slprp_def(n), n
+        if got and not _isprime(n):
+            passing_composites.append(n)
+    assert passing_composites == [5459, 5777, 10877, 16109, 18971, 22499, 24569, 25199]
+
+
+def test_strong_lucas_64bit(emu_lucas):
+    """Large n against sympy's is_strong_lucas_prp (a library routine) and the matrix
+    definition: random odd n, primes near 2^64, squares, semiprimes with large factors."""
+    from sympy.ntheory.primetest import is_strong_lucas_prp
+    rng = random.Random(37)
+    ns = [rng.getrandbits(64) | 1 for _ in range(1500)] + [rng.getrandbits(rng.randint(20, 63)) | 1 for _ in range(1500)]
+    ns += [(1 << 64) - 59, (1 << 64) - 83, (1 << 64) - 1, (1 << 63) + 29, 4294967291 ** 2, 4294967279 * 4294967291,
+           3825123056546413051, 1000000007 * 998244353]
+    ns += [q for q in (rng.randrange(1 << 62, 1 << 64) | 1 for _ in range(4000)) if _isprime(q)][:300]
+    for n in ns:
+        want = is_strong_lucas_prp(n)
+        assert bool(emu_lucas.emu_slprp(n)) == want, n
+        if n < (1 << 40):
+            assert _slprp_def(n) == want, n
+
+
+def test_bpsw_flags(emu_lucas):
+    """bit 1: base-2 strong probable prime, bit 2: strong Lucas probable prime, bit 4: prime.
+    Base-2 strong pseudoprimes set only bit 1, strong Lucas pseudoprimes only bit 2, and the
+    verdict agrees with sympy.isprime on random 64-bit numbers and on every n < 20000."""
+    import sympy
+    for n in (2047, 3277, 4033, 4681, 8321, 3825123056546413051, 18446744066047760377):
+        assert emu_lucas.emu_bpsw(n) == 1, n
+    for n in (5459, 5777, 10877, 16109, 18971):
+        assert emu_lucas.emu_bpsw(n) == 2, n
+    assert [emu_lucas.emu_bpsw(n) for n in (0, 1, 2, 3, 4, 5, 9)] == [0, 0, 4, 7, 0, 7, 0]
+    for n in range(0, 20000):
+        assert bool(emu_lucas.emu_bpsw(n) & 4) == sympy.isprime(n), n
+    rng = random.Random(41)
+    for _ in range(3000):
+        n = rng.getrandbits(64) | 1
+        assert bool(emu_lucas.emu_bpsw(n) & 4) == sympy.isprime(n), n

@@ -197,3 +197,27 @@ def test_check_known_bases():
     rc, out, err = run("check", "1006003")
     assert rc == 0 and "base   3: q = 0  A = 0  <-- WIEFERICH  [reference agrees]" in out, out
     assert "DISAGREES" not in out
+
+
+@pytest.mark.gpu
+def test_num_command():
+    """wf num: the GPU's base-2 strong test, strong Lucas test and verdict per number."""
+    rc, out, err = run("num", "2047", "5459", "1000000007", "18446744073709551557", "18446744073709551615")
+    assert rc == 0, err
+    assert out.splitlines() == ["2047: gpu sprp2 1 slprp 0 prime 0", "5459: gpu sprp2 0 slprp 1 prime 0",
+                                "1000000007: gpu sprp2 1 slprp 1 prime 1",
+                                "18446744073709551557: gpu sprp2 1 slprp 1 prime 1",
+                                "18446744073709551615: gpu sprp2 0 slprp 0 prime 0"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["on", "off"])
+def test_scan_prove_modes_same_output(tmp_path, mode):
+    """--prove on / off: the same SUMMARY and chunk log as the default."""
+    outs = []
+    for m in ("auto", mode):
+        log = tmp_path / ("log_" + m)
+        rc, out, err = run("scan", "1e17", "1e17+4e6", "-c", "1e6", "-n", "1000", "-L", log, "-q", "-Q", "--prove", m)
+        assert rc == 0, err
+        outs.append((out, log.read_text()))
+    assert outs[0] == outs[1]
