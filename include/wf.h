@@ -20,6 +20,9 @@
  *                  (the prime set of [lo, hi), for tests and the sieve benchmark)
  *   wf_bpsw        d_sprp2_64 + d_slprp64 of A350826/cuda/a350826_cuda.cu:768-898 (its
  *                  test_kernel / `num` command, :2458-2483)
+ *   wf_sieve_stats the GPU sieve for the repository's other prime consumers: twin counts
+ *                  (A007508/README.md:3-5, 44-64) and runs of increasing gaps
+ *                  (A158939/a158939.c:19-24, A158939/cuda/a158939_cuda.cu:53-78)
  *   wf_serve_*     (no counterpart: a device-resident request loop)
  *
  * Conventions (as wieferich.c): return 0 on success or a negative WF_E_* code
@@ -134,6 +137,22 @@ int  wf_quotients_base(wf_engine *e, uint32_t b, const uint64_t *p, uint64_t *q,
 /* The primes of [lo, hi) in ascending order into out[0..cap); *count = how many exist. */
 int  wf_primes(wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *out, uint64_t cap,
                uint64_t *count);
+
+/* Statistics of the primes of [lo, hi) for the repository's other sieve consumers.
+ * L(p) is A158939's run length: with q_0 = p < q_1 < q_2 < ... the primes from p on and
+ * g_i = q_i - q_(i-1), L(p) = the largest n with g_1 < g_2 < ... < g_n (always >= 1). */
+typedef struct {
+    uint64_t primes;            /* primes p in [lo, hi) */
+    uint64_t twins;             /* p in [lo, hi) with p + 2 prime (A007508: pairs below hi) */
+    uint64_t max_gap;           /* largest gap q_1 - p over p in [lo, hi) */
+    uint64_t max_gap_p;         /* the smallest p with that gap */
+    uint64_t runs[64];          /* runs[n] = primes p in [lo, hi) with L(p) = n */
+    uint64_t first[64];         /* the smallest of them (0 if none) */
+} wf_stats;
+
+/* The gaps and runs after a prime near hi use the primes beyond it: hi <= WF_STATS_MAXHI. */
+#define WF_STATS_MAXHI (0xFFFFFFFFFFFFFFFFull - (1ull << 20) + 1)
+int  wf_sieve_stats(wf_engine *e, uint64_t lo, uint64_t hi, wf_stats *out);
 
 /* BPSW verdicts of n[0..count): flags[i] bit 0 = base-2 strong probable prime, bit 1 =
  * strong Lucas probable prime with Selfridge's parameters (both for odd n >= 3), bit 2 =

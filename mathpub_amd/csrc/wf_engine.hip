@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -941,6 +942,90 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
   }
 }
 
+// ---------------------------------------------------------------------------------
+// prime statistics for the other sieve consumers (twin pairs, runs of increasing gaps)
+// ---------------------------------------------------------------------------------
+//
+// The segment lists of a pass are ascending and the segments consecutive, so the primes
+// after p are read straight from them.  A lane takes one prime p of [a, b) and walks the
+// following primes while the gaps increase: L(p) = the number of increasing gaps
+// (A158939/a158939.c:19-24); the first gap also gives the twin flag (A007508) and the gap
+// maximum.  The pass sieves STATS_OV numbers past b so that every walk ends inside it.
+#define STATS_OV (1ull << 20)
+#define STATS_MAXRUN 64
+
+struct StatsOut {
+  unsigned long long primes, twins, gapkey;   // gapkey: gap << 48 | (2^48 - 1 - (p - a)), maximised
+  unsigned long long runs[STATS_MAXRUN];
+  unsigned long long first[STATS_MAXRUN];
+};
+
+__global__ void __launch_bounds__(256) k_stats(const SegDesc* __restrict__ segs, u32 nseg, const u32* __restrict__ off,
+                                               const u32* __restrict__ cnt, u32 pcap, u32 tps, u64 a, u64 b,
+                                               StatsOut* __restrict__ out, u32* __restrict__ flags) {
+  __shared__ unsigned long long hist[STATS_MAXRUN], first[STATS_MAXRUN];
+  if (threadIdx.x < STATS_MAXRUN) {
+    hist[threadIdx.x] = 0;
+    first[threadIdx.x] = ~0ull;
+  }
+  __syncthreads();
+  const u64 M48 = (1ull << 48) - 1;
+  u64 np = 0, ntw = 0, gk = 0;
+  const u64 nitems = (u64)nseg * tps;
+  for (u64 t = blockIdx.x; t < nitems; t += gridDim.x) {
+    const u32 s = (u32)(t / tps);
+    const u32 i = (u32)(t - (u64)s * tps) * 256u + threadIdx.x;
+    if (i >= cnt[s]) continue;
+    const u64 p = segs[s].base + off[(u64)s * pcap + i];
+    if (p < a || p >= b) continue;
+    np++;
+    u32 cs = s, ci = i;
+    u64 prev = p, gprev = 0;
+    int L = 0;
+    bool ended = false;
+    for (int step = 0; step < STATS_MAXRUN; step++) {
+      ci++;
+      while (cs < nseg && ci >= cnt[cs]) {          // the next prime, in a later segment if need be
+        cs++;
+        ci = 0;
+      }
+      if (cs >= nseg) break;                          // past the sieved overlap: reported below
+      const u64 q = segs[cs].base + off[(u64)cs * pcap + ci];
+      const u64 g = q - prev;
+      if (step == 0) {
+        ntw += g == 2;
+        gk = max(gk, (g << 48) | (M48 - (p - a)));
+      }
+      if (g <= gprev) {
+        ended = true;
+        break;
+      }
+      L++;
+      gprev = g;
+      prev = q;
+    }
+    if (!ended) {
+      atomicOr(flags, 4u);
+      continue;
+    }
+    atomicAdd(&hist[L], 1ull);
+    atomicMin(&first[L], (unsigned long long)p);
+  }
+  np = wave_sum(np);
+  ntw = wave_sum(ntw);
+  for (int o = 32; o > 0; o >>= 1) gk = max(gk, (u64)__shfl_xor((unsigned long long)gk, o, 64));
+  if ((threadIdx.x & 63) == 0) {
+    if (np) atomicAdd(&out->primes, (unsigned long long)np);
+    if (ntw) atomicAdd(&out->twins, (unsigned long long)ntw);
+    if (gk) atomicMax(&out->gapkey, (unsigned long long)gk);
+  }
+  __syncthreads();
+  if (threadIdx.x < STATS_MAXRUN && hist[threadIdx.x]) {
+    atomicAdd(&out->runs[threadIdx.x], hist[threadIdx.x]);
+    atomicMin(&out->first[threadIdx.x], first[threadIdx.x]);
+  }
+}
+
 // BPSW verdicts of arbitrary n (bits: 1 base-2 strong probable prime, 2 strong Lucas
 // probable prime, 4 prime).
 __global__ void k_bpsw(const u64* __restrict__ n_in, unsigned char* __restrict__ out, u64 n) {
@@ -1013,7 +1098,8 @@ struct wf_engine {
   u64 acc_cap = 0;
   DevEvent* d_ev = nullptr;
   u64 dev_ev_cap = 0;
-  unsigned long long* d_ctr = nullptr;   // [1] events, [2] strong pseudoprimes
+  StatsOut* d_stats = nullptr;           // wf_sieve_stats totals
+  unsigned long long* d_ctr = nullptr;   // [1] events, [2] strong pseudoprimes, [3] proved primes
   u64* d_psp = nullptr;                  // strong pseudoprimes found
   u64 psp_cap = 0;
   hipEvent_t ev[4] = {};
@@ -1269,6 +1355,7 @@ extern "C" void wf_close(wf_engine* e) {
     (void)hipFree(e->bs[k].tcnt);
   }
   (void)hipFree(e->d_psp);
+  (void)hipFree(e->d_stats);
   (void)hipFree(e->d_pre);
   (void)hipFree(e->d_flags);
   (void)hipFree(e->d_acc);
@@ -1995,6 +2082,70 @@ extern "C" int wf_bpsw(wf_engine* e, const u64* n, uint8_t* flags, u64 count) {
     rc = WF_E_DEVICE;
   (void)hipFree(d);
   return rc;
+}
+
+extern "C" int wf_sieve_stats(wf_engine* e, u64 lo, u64 hi, wf_stats* out) {
+  if (!e || !out || hi <= lo || hi > WF_STATS_MAXHI) return WF_E_ARG;
+  HIPCHK(hipSetDevice(e->dev));
+  int rc;
+  if ((rc = ensure_sieving_primes(e, isqrt_u64(hi - 1 + STATS_OV) + 1)) != WF_OK) return rc;
+  if ((rc = grow(e->d_acc, e->acc_cap, 1)) != WF_OK) return rc;
+  if (!e->d_stats && hipMalloc(&e->d_stats, sizeof(StatsOut)) != hipSuccess) return WF_E_NOMEM;
+  HIPCHK(hipMemsetAsync(e->d_stats, 0, offsetof(StatsOut, first), e->st));
+  HIPCHK(hipMemsetAsync(e->d_stats->first, 0xFF, sizeof e->d_stats->first, e->st));
+  HIPCHK(hipMemsetAsync(e->d_flags, 0, sizeof(u32), e->st));
+  HIPCHK(hipMemsetAsync(e->d_ctr + 3, 0, sizeof(unsigned long long), e->st));
+  memset(out, 0, sizeof *out);
+  const u64 segspan = 30ull * e->cfg.seg_bytes;
+  // passes of at most cfg.chunk numbers (and < 2^46, for the gap key's offset field)
+  const u64 step = std::max<u64>(1, std::min<u64>(e->cfg.chunk, 1ull << 46) / segspan) * segspan;
+  const int cus = e->prop.multiProcessorCount;
+  for (u64 a = lo; a < hi;) {
+    const u64 b = hi - a <= step ? hi : a + step;
+    std::vector<Piece> pieces = {{a, b + STATS_OV, 0, A_F64}};
+    PassPlan P;
+    plan_pass(e, pieces, P);
+    if ((rc = size_set(e->bs[0], P)) != WF_OK) return rc;
+    if ((rc = upload_pass(e, P)) != WF_OK) return rc;
+    if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->d_fills, e->bs[0], e->st, e->d_acc)) != WF_OK) return rc;
+    HIPCHK(hipMemsetAsync(&e->d_stats->gapkey, 0, sizeof(unsigned long long), e->st));
+    const u32 tps = (P.pcap + 255) / 256;
+    const u64 nseg = P.segs.size();
+    hipLaunchKernelGGL(k_stats, dim3((u32)std::min<u64>(nseg * tps, (u64)cus * 16)), dim3(256), 0, e->st, e->d_segs,
+                       (u32)nseg, e->bs[0].off, e->bs[0].cnt, P.pcap, tps, a, b, e->d_stats, e->d_flags);
+    HIPCHK(hipGetLastError());
+    unsigned long long gk = 0;
+    u32 flags = 0;
+    HIPCHK(hipMemcpyAsync(&gk, &e->d_stats->gapkey, sizeof gk, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipMemcpyAsync(&flags, e->d_flags, sizeof flags, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));   // also keeps P's host arrays alive for the uploads
+    if (flags) return WF_E_OVERFLOW;
+    const u64 gap = gk >> 48, gp = a + (((1ull << 48) - 1) - (gk & ((1ull << 48) - 1)));
+    if (gk && gap > out->max_gap) {       // ties keep the earlier pass's smaller p
+      out->max_gap = gap;
+      out->max_gap_p = gp;
+    }
+    a = b;
+  }
+  StatsOut h;
+  HIPCHK(hipMemcpy(&h, e->d_stats, sizeof h, hipMemcpyDeviceToHost));
+  out->primes = h.primes;
+  out->twins = h.twins;
+  for (int n = 0; n < 64; n++) {
+    out->runs[n] = h.runs[n];
+    out->first[n] = h.first[n] == ~0ull ? 0 : h.first[n];
+  }
+  // the prime 2 is not on the wheel: L(2) = 2 (gaps 1 < 2, then 2), its gap is 1
+  if (lo <= 2 && hi > 2) {
+    out->primes++;
+    out->runs[2]++;
+    out->first[2] = 2;
+    if (out->max_gap < 1) {
+      out->max_gap = 1;
+      out->max_gap_p = 2;
+    }
+  }
+  return WF_OK;
 }
 
 extern "C" int wf_primes(wf_engine* e, u64 lo, u64 hi, u64* out, u64 cap, u64* count) {

@@ -7,6 +7,9 @@
  *   wf check P [-b BASES] [-g DEV]
  *   wf bench [N] [-d SPAN] [-b BASES] [-g DEV]
  *   wf selftest [LIMIT] [-g DEV]
+ *   wf num X [X ...] [-g DEV]                     (BPSW tests, as a350826_cuda num)
+ *   wf dcheck CHUNKLOG BASES [STEP] [THREADS] [OUT] [PID]    (as A001220/dcheck.sh)
+ *   wf stats [LO] HI [-g DEV]                     (twin pairs, gaps, runs of increasing gaps)
  *
  * Kept byte for byte from the reference: the number grammar (wieferich.c:300-416), the
  * base-list grammar (:816-849), chunking with END rounded up to a chunk boundary
@@ -328,6 +331,17 @@ static void print_u128(FILE *f, u128 v)
     else fprintf(f, "%" PRIu64, (u64)v);
 }
 
+/* The chunk-log line of [lo, hi) without its newline: "C lo hi primes b:cks ..."
+ * (wieferich.c:1166-1172). */
+static void format_chunk_line(char *buf, size_t n, u64 lo, u128 hi, u64 primes, const u64 *cks)
+{
+    size_t k = (size_t)snprintf(buf, n, "C %" PRIu64 " ", lo);
+    if (hi >> 64) k += (size_t)snprintf(buf + k, n - k, "18446744073709551616");
+    else k += (size_t)snprintf(buf + k, n - k, "%" PRIu64, (u64)hi);
+    k += (size_t)snprintf(buf + k, n - k, " %" PRIu64, primes);
+    for (int j = 0; j < g_nb && k < n; j++) k += (size_t)snprintf(buf + k, n - k, " %u:%016" PRIx64, g_b[j], cks[j]);
+}
+
 static void record_hit(u32 b, u64 p, u64 idx)
 {
     static bool warned = false;
@@ -490,11 +504,9 @@ static void fold_chunk(u64 idx, const wf_result *r, const wf_event *ev, u64 nev)
     S.flt += r->flt;
     for (int j = 0; j < g_nb; j++) { S.cks[j] += r->cks[j]; S.sumq[j] += r->sumq[j]; }
     if (g_log) {
-        fprintf(g_log, "C %" PRIu64 " ", chunk_lo(idx));
-        print_u128(g_log, chunk_hi(idx));
-        fprintf(g_log, " %" PRIu64, r->primes);
-        for (int j = 0; j < g_nb; j++) fprintf(g_log, " %u:%016" PRIx64, g_b[j], r->cks[j]);
-        fputc('\n', g_log);
+        char line[64 + WF_MAXB * 32];
+        format_chunk_line(line, sizeof line, chunk_lo(idx), chunk_hi(idx), r->primes, r->cks);
+        fprintf(g_log, "%s\n", line);
         fflush(g_log);
     }
     S.frontier++;
@@ -631,6 +643,8 @@ static void usage(void)
           "       wf bench [N] [-d SPAN] [-b BASES] [-g DEV]\n"
           "       wf selftest [LIMIT] [-g DEV]\n"
           "       wf num X [X ...] [-g DEV]          BPSW tests of single numbers on the GPU\n"
+          "       wf dcheck CHUNKLOG BASES [STEP] [THREADS] [OUT] [PID] [-g DEV]   recompute every STEP-th chunk\n"
+          "       wf stats [LO] HI [-g DEV]          primes, twin pairs, largest gap, runs of increasing gaps\n"
           "scan --prove auto|off|on: sieve short chunks with small primes only and prove the rest by BPSW\n"
           "BASES: list/ranges like 3,5,7 or 2-30; 'oeis' (default) = 3,5,6,7,10,12,13,14,15,17,18,19,20,22,23,26,30;\n"
           "       'known' = the bases 2..30 that are not perfect powers (default of check)\n"
@@ -1063,6 +1077,144 @@ static int cmd_num(int argc, char **argv)
     return 0;
 }
 
+/* dcheck CHUNKLOG BASES [STEP] [THREADS] [OUT] [PID]: A001220/dcheck.sh on the GPU.
+ * Every STEP-th line (default 16) of a chunk log is recomputed with an engine set up
+ * differently from a scan's: prove mode for every piece (small-prime sieve + BPSW instead
+ * of the bucket sieve), 16 KiB segments instead of 64 KiB, and for the single base 2 the
+ * 32-bit limb Montgomery ladders instead of the FP64 p-adic / radix-2^28 arithmetic.  The
+ * recomputed line must equal the logged one byte for byte; "MATCH lo hi TIME" or
+ * "MISMATCH lo hi gpu: LINE check: LINE" is appended to OUT (default CHUNKLOG.dcheck).
+ * Chunks already in OUT are skipped, so a run can be resumed; with PID, new chunks are
+ * awaited while that process lives (dcheck.sh:12-45).  THREADS is accepted and ignored.
+ * Exit status 0 when every chunk checked in this run matched, 1 otherwise. */
+static bool out_has(const char *out, u64 lo, const char *hi)
+{
+    FILE *f = fopen(out, "r");
+    if (!f) return false;
+    char key[64], line[8192];
+    snprintf(key, sizeof key, " %" PRIu64 " %s ", lo, hi);
+    bool found = false;
+    while (!found && fgets(line, sizeof line, f)) found = strstr(line, key) != NULL;
+    fclose(f);
+    return found;
+}
+
+static int cmd_dcheck(int argc, char **argv)
+{
+    const char *pos[6] = {0};
+    int npos = 0;
+    for (int i = 0; i < argc; i++) {
+        if (!strcmp(argv[i], "-g") && i + 1 < argc) g_dev = parse_int(argv[++i], 0, 64, "device");
+        else if (argv[i][0] != '-' && npos < 6) pos[npos++] = argv[i];
+        else usage();
+    }
+    if (npos < 2) usage();
+    const char *logp = pos[0];
+    parse_bases(pos[1]);
+    const u64 step = npos > 2 ? parse_num(pos[2]) : 16;
+    if (step == 0) die("STEP must be positive");
+    if (npos > 3) (void)parse_int(pos[3], 1, 1024, "threads");
+    char outbuf[4096];
+    snprintf(outbuf, sizeof outbuf, "%s.dcheck", logp);
+    const char *outp = npos > 4 ? pos[4] : outbuf;
+    const long pid = npos > 5 ? (long)parse_num(pos[5]) : 0;
+    g_cfg.prove = WF_PROVE_ON;
+    g_cfg.seg_bytes = 16384;
+    if (g_nb == 1 && g_b[0] == 2) g_cfg.arith = 1;
+    engine_open();
+    FILE *lf = fopen(logp, "r");
+    if (!lf) die("cannot open %s: %s", logp, strerror(errno));
+    const u64 EVCAP = 1 << 16;
+    wf_event *ev = malloc(EVCAP * sizeof *ev);
+    if (!ev) die("out of memory");
+    u64 k = 0, checked = 0, bad = 0;
+    char line[64 + WF_MAXB * 32], mine[64 + WF_MAXB * 32];
+    for (;;) {
+        bool progressed = false;
+        long where = ftell(lf);
+        while (fgets(line, sizeof line, lf)) {
+            size_t n = strlen(line);
+            if (n == 0 || line[n - 1] != '\n') {           /* a line still being written */
+                fseek(lf, where, SEEK_SET);
+                break;
+            }
+            line[--n] = 0;
+            where = ftell(lf);
+            k++;
+            if (k % step) continue;
+            char los[32], his[32];
+            if (sscanf(line, "C %31s %31s", los, his) != 2) die("%s: not a chunk-log line: %s", logp, line);
+            const u64 lo = parse_num(los);
+            const u128 hi = parse_num128(his);
+            if (hi <= lo || hi > LIMIT128) die("%s: bad range in: %s", logp, line);
+            if (out_has(outp, lo, his)) continue;
+            wf_result r;
+            int rc = wf_scan(g_eng, lo, hi >= LIMIT128 ? 0 : (u64)hi, 0, &r, ev, EVCAP, 0, NULL, 0);
+            if (rc) die("chunk [%" PRIu64 ", %s): %s", lo, his, wf_strerror(rc));
+            if (r.flt) snprintf(mine, sizeof mine, "BAD range [%" PRIu64 ", %s) flt %" PRIu64, lo, his, r.flt);
+            else format_chunk_line(mine, sizeof mine, lo, hi, r.primes, r.cks);
+            FILE *of = fopen(outp, "a");
+            if (!of) die("cannot open %s: %s", outp, strerror(errno));
+            if (!strcmp(mine, line)) {
+                char ts[32];
+                const time_t t = time(NULL);
+                strftime(ts, sizeof ts, "%Y-%m-%dT%H:%M:%S", localtime(&t));
+                fprintf(of, "MATCH %" PRIu64 " %s %s\n", lo, his, ts);
+            } else {
+                fprintf(of, "MISMATCH %" PRIu64 " %s gpu: %s check: %s\n", lo, his, line, mine);
+                bad++;
+            }
+            fclose(of);
+            checked++;
+            progressed = true;
+        }
+        clearerr(lf);
+        if (pid > 0 && kill((pid_t)pid, 0) == 0) {
+            if (!progressed) sleep(60);
+        } else if (!progressed) {
+            break;
+        }
+    }
+    fclose(lf);
+    free(ev);
+    wf_close(g_eng);
+    fprintf(stderr, "dcheck: %" PRIu64 " chunks checked, %" PRIu64 " mismatches\n", checked, bad);
+    return bad ? 1 : 0;
+}
+
+/* stats [LO] HI: the sieve's statistics for the repository's other prime consumers: the
+ * twin pairs of A007508 (p, p + 2 both prime, p < HI), the largest gap, and the run lengths
+ * of A158939 (L(p) = number of strictly increasing gaps after p, A158939/a158939.c:19-24)
+ * with the first prime of each. */
+static int cmd_stats(int argc, char **argv)
+{
+    u128 pos[2];
+    int npos = 0;
+    for (int i = 0; i < argc; i++) {
+        if (!strcmp(argv[i], "-g") && i + 1 < argc) g_dev = parse_int(argv[++i], 0, 64, "device");
+        else if (argv[i][0] != '-' && npos < 2) pos[npos++] = parse_num128(argv[i]);
+        else usage();
+    }
+    if (npos == 0) usage();
+    const u128 lo = npos == 2 ? pos[0] : 0, hi = npos == 2 ? pos[1] : pos[0];
+    if (hi <= lo) die("END must be larger than START");
+    if (hi > (u128)WF_STATS_MAXHI) die("END must be <= 2^64 - 2^20 (the gaps after the last primes are needed)");
+    g_b[0] = 2;
+    g_nb = 1;
+    engine_open();
+    wf_stats st;
+    const double t0 = now();
+    const int rc = wf_sieve_stats(g_eng, (u64)lo, (u64)hi, &st);
+    if (rc) die("wf_sieve_stats: %s", wf_strerror(rc));
+    printf("STATS [%" PRIu64 ", %" PRIu64 ") primes %" PRIu64 " twins %" PRIu64 " max_gap %" PRIu64 " after %" PRIu64 "\n",
+           (u64)lo, (u64)hi, st.primes, st.twins, st.max_gap, st.max_gap_p);
+    for (int n = 1; n < 64; n++)
+        if (st.runs[n]) printf("run %2d count %" PRIu64 " first %" PRIu64 "\n", n, st.runs[n], st.first[n]);
+    fprintf(stderr, "%.3f s\n", now() - t0);
+    wf_close(g_eng);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     stderr_tty = isatty(2);
@@ -1073,6 +1225,8 @@ int main(int argc, char **argv)
     if (!strcmp(cmd, "bench")) return cmd_bench(argc - 2, argv + 2);
     if (!strcmp(cmd, "selftest")) return cmd_selftest(argc - 2, argv + 2);
     if (!strcmp(cmd, "num")) return cmd_num(argc - 2, argv + 2);
+    if (!strcmp(cmd, "dcheck")) return cmd_dcheck(argc - 2, argv + 2);
+    if (!strcmp(cmd, "stats")) return cmd_stats(argc - 2, argv + 2);
     usage();
     return 2;
 }

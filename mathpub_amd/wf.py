@@ -26,6 +26,14 @@ class Result(ctypes.Structure):
                 ("sumq", u64 * MAXB), ("nev", u64), ("npsp", u64), ("nbpsw", u64)]
 
 
+class SieveStats(ctypes.Structure):
+    _fields_ = [("primes", u64), ("twins", u64), ("max_gap", u64), ("max_gap_p", u64), ("runs", u64 * 64),
+                ("first", u64 * 64)]
+
+
+STATS_MAXHI = (1 << 64) - (1 << 20)
+
+
 class Range(ctypes.Structure):
     _fields_ = [("lo", u64), ("hi", u64)]
 
@@ -73,6 +81,7 @@ def load():
     L.wf_quotients_base.argtypes = [ctypes.c_void_p, u32, P64, P64, P64, P64, u64]
     L.wf_primes.argtypes = [ctypes.c_void_p, u64, u64, P64, u64, P64]
     L.wf_bpsw.argtypes = [ctypes.c_void_p, P64, ctypes.POINTER(ctypes.c_uint8), u64]
+    L.wf_sieve_stats.argtypes = [ctypes.c_void_p, u64, u64, ctypes.POINTER(SieveStats)]
     L.wf_last_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(Timing)]
     L.wf_autotune.argtypes = [ctypes.c_void_p, u64, u64, ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_double)]
     L.wf_serve_start.argtypes = [ctypes.c_void_p, u32, u64]
@@ -80,7 +89,7 @@ def load():
     L.wf_serve_poll.argtypes = [ctypes.c_void_p, u64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_int)]
     L.wf_serve_stop.argtypes = [ctypes.c_void_p]
     for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_quotients_base",
-              "wf_primes", "wf_bpsw",
+              "wf_primes", "wf_bpsw", "wf_sieve_stats",
               "wf_last_timing", "wf_autotune", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
         getattr(L, f).restype = ctypes.c_int
     _lib = L
@@ -195,6 +204,16 @@ class Engine:
         out = (ctypes.c_uint8 * max(n, 1))()
         _check(self._L.wf_bpsw(self._h, arr, out, n), "wf_bpsw")
         return list(out[:n])
+
+    def sieve_stats(self, lo, hi):
+        """Primes, twin pairs, largest gap and A158939 run lengths of the primes of [lo, hi)
+        (wf_sieve_stats): runs / first map n to the count and the smallest p with L(p) = n."""
+        st = SieveStats()
+        _check(self._L.wf_sieve_stats(self._h, lo, hi, ctypes.byref(st)), "wf_sieve_stats")
+        runs = {n: st.runs[n] for n in range(64) if st.runs[n]}
+        first = {n: st.first[n] for n in runs}
+        return {"primes": st.primes, "twins": st.twins, "max_gap": st.max_gap, "max_gap_p": st.max_gap_p,
+                "runs": runs, "first": first}
 
     def autotune(self, lo, hi):
         """Pick segment size, sieve workgroup and bucket threshold for [lo, hi) (wf_autotune)."""

@@ -151,3 +151,35 @@ def scan(lo, hi, bases=(2,), near_T=0, threads=None, ev_cap=1 << 16, psp=False, 
         "nev": res.nev, "events": events,
         "npsp": res.npsp, "psp": list(pspbuf[:min(res.npsp, psp_cap)]),
     }
+
+
+def sieve_stats(lo, hi, ov=1 << 20):
+    """Statistics of the primes of [lo, hi) from the oracle's prime list of [lo, hi + ov):
+
+    * twins: p in [lo, hi) with p + 2 prime -- A007508's pairs (A007508/README.md:3-5);
+    * run lengths L(p) = the largest n with g_1 < g_2 < ... < g_n, g_i the gaps after p
+      (A158939/a158939.c:19-24): L(p_i) = 1 + L(p_(i+1)) if g_(i+1) > g_i, else 1, evaluated
+      here from the positions where the gap sequence stops increasing;
+    * the largest gap after a prime of [lo, hi) and the first p with it.
+    Returns the dict of mathpub_amd.wf.Engine.sieve_stats."""
+    import numpy as np
+    ps = np.array(([2] if lo <= 2 < hi + ov else []) + primes(max(lo, 3), hi + ov), dtype=np.uint64)
+    ps = ps[ps >= lo]
+    n_in = int(np.searchsorted(ps, np.uint64(hi)))
+    g = np.diff(ps).astype(np.int64)                      # g[i] = gap after ps[i]
+    inc = g[1:] > g[:-1]                                  # inc[i]: the run from ps[i] goes on
+    stops = np.flatnonzero(~inc)                          # runs end where inc is False
+    if n_in and (len(stops) == 0 or stops[-1] < n_in - 1):
+        raise ValueError("overlap too short for the last runs")
+    nxt = stops[np.searchsorted(stops, np.arange(n_in))]  # first stop at or after i
+    L = (nxt - np.arange(n_in) + 1).astype(np.int64)
+    runs, first = {}, {}
+    for n in np.unique(L):
+        sel = np.flatnonzero(L == n)
+        runs[int(n)] = int(len(sel))
+        first[int(n)] = int(ps[sel[0]])
+    gi = g[:n_in]
+    k = int(np.argmax(gi)) if n_in else 0
+    return {"primes": n_in, "twins": int(np.count_nonzero(gi == 2)),
+            "max_gap": int(gi[k]) if n_in else 0, "max_gap_p": int(ps[k]) if n_in else 0,
+            "runs": runs, "first": first}

@@ -221,3 +221,55 @@ def test_scan_prove_modes_same_output(tmp_path, mode):
         assert rc == 0, err
         outs.append((out, log.read_text()))
     assert outs[0] == outs[1]
+
+
+@pytestmark_cpu
+def test_dcheck_usage():
+    rc, out, err = run("dcheck")
+    assert rc == 2 and "usage" in err
+    rc, out, err = run("dcheck", "/nonexistent/log", "2", "0")
+    assert rc == 1 and "STEP must be positive" in err
+
+
+@pytest.mark.gpu
+def test_dcheck_matches_then_flags_a_bad_line(tmp_path):
+    """wf dcheck (dcheck.sh's double check on the GPU, with prove mode, 16 KiB segments and the
+    32-bit ladders): every chunk of a scan's log matches; a line with one wrong checksum digit
+    is reported as MISMATCH; a second run over the same OUT checks nothing again."""
+    log = tmp_path / "log"
+    rc, out, err = run("scan", "1e15", "1e15+6e7", "-c", "1e7", "-b", "2", "-L", log, "-q")
+    assert rc == 0, err
+    lines = log.read_text().splitlines()
+    assert len(lines) == 6
+    out1 = tmp_path / "out1"
+    rc, out, err = run("dcheck", log, "2", "1", "8", out1)
+    assert rc == 0, err
+    res = out1.read_text().splitlines()
+    assert [r.split()[:3] for r in res] == [["MATCH"] + l.split()[1:3] for l in lines]
+    rc, out, err = run("dcheck", log, "2", "1", "8", out1)       # resumed: nothing left to check
+    assert rc == 0 and "0 chunks checked" in err and len(out1.read_text().splitlines()) == 6
+    bad = list(lines)
+    last = bad[3][-1]
+    bad[3] = bad[3][:-1] + ("0" if last != "0" else "1")
+    badlog = tmp_path / "badlog"
+    badlog.write_text("\n".join(bad) + "\n")
+    out2 = tmp_path / "out2"
+    rc, out, err = run("dcheck", badlog, "2", "2", "8", out2)     # every 2nd chunk: lines 2, 4, 6
+    assert rc == 1
+    res = out2.read_text().splitlines()
+    assert [r.split()[0] for r in res] == ["MATCH", "MISMATCH", "MATCH"]
+    assert res[1].startswith("MISMATCH %s %s gpu: %s check: %s" % (bad[3].split()[1], bad[3].split()[2], bad[3],
+                                                                    lines[3]))
+
+
+@pytest.mark.gpu
+def test_dcheck_other_bases_and_top_of_word(tmp_path):
+    """The 'oeis' base list at 1e12 and base 2 just below 2^64 (END = 2^64 in the log)."""
+    for start, end, chunk, bases in (("1e12", "1e12+4e6", "2e6", "oeis"), ("2^64-4e6", "2^64", "2e6", "2")):
+        log = tmp_path / ("log" + bases)
+        rc, out, err = run("scan", start, end, "-c", chunk, "-b", bases, "-L", log, "-q")
+        assert rc == 0, err
+        outp = tmp_path / ("out" + bases)
+        rc, out, err = run("dcheck", log, bases, "1", "1", outp)
+        assert rc == 0, err
+        assert [r.split()[0] for r in outp.read_text().splitlines()] == ["MATCH", "MATCH"]

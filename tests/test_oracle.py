@@ -187,3 +187,53 @@ def test_sumq_and_near_small_window():
         assert r["sumq"][b] == s & M64
         got = [(e[2], e[3] if e[3] <= e[2] // 2 else e[3] - e[2]) for e in r["events"] if e[0] == 1 and e[1] == b]
         assert got == near
+
+
+# ---- statistics for the other sieve consumers (twin pairs, runs of increasing gaps) ----
+
+def _stats_brute(lo, hi):
+    """From the definitions, over a plain list of primes (sympy): twins, L(p) by walking."""
+    import sympy
+    ps = list(sympy.primerange(0, hi + 3000))
+    runs, first, tw, best = {}, {}, 0, (0, 0)
+    for i, p in enumerate(ps):
+        if p < lo or p >= hi:
+            continue
+        g0 = ps[i + 1] - p
+        tw += g0 == 2
+        if g0 > best[0]:
+            best = (g0, p)
+        L, gp, j = 0, 0, i
+        while ps[j + 1] - ps[j] > gp:
+            gp = ps[j + 1] - ps[j]
+            L += 1
+            j += 1
+        runs[L] = runs.get(L, 0) + 1
+        first.setdefault(L, p)
+    return {"twins": tw, "runs": runs, "first": first, "max_gap": best[0], "max_gap_p": best[1]}
+
+
+@pytest.mark.parametrize("lo,hi", [(0, 3), (0, 8), (2, 3), (3, 100), (0, 200000), (123457, 300001)])
+def test_sieve_stats_brute_force(lo, hi):
+    s = oracle.sieve_stats(lo, hi)
+    b = _stats_brute(lo, hi)
+    for k in ("twins", "runs", "first", "max_gap", "max_gap_p"):
+        assert s[k] == b[k], (k, s[k], b[k])
+
+
+def test_sieve_stats_reference_data():
+    """[0, 10^8): the twin pairs below 10^n (A007508/README.md) for n <= 8 and the first
+    primes a(1)..a(10) of A158939 (A158939/README.md), with the gaps the README lists after
+    them (the last one ends the run)."""
+    s = oracle.sieve_stats(0, 10 ** 8)
+    assert s["primes"] == 5761455
+    a7508 = FACTS["a007508"]
+    for n in range(1, 9):
+        assert oracle.sieve_stats(0, 10 ** n)["twins"] == a7508[n - 1], n
+    for n in range(1, 11):
+        a = FACTS["a158939"][str(n)]
+        assert s["first"][n] == a, n
+        ps = oracle.primes(a, a + 10 ** 4)
+        gaps = [ps[i + 1] - ps[i] for i in range(n + 1)]
+        assert gaps == FACTS["a158939_gaps"][str(n)], n
+    assert max(s["runs"]) == 10
