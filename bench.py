@@ -215,6 +215,61 @@ def bench_published(args):
                          "data": "synthetic (the range is the input; primes sieved on the GPU)"})
 
 
+def bench_prove(args):
+    """Prove mode against the full sieve: sieve-stage device ms of one range per call
+    (fill + sieve + prove, wf_last_timing) for lengths 1e5..1e9 at heights 1e13..2^64, with
+    prove OFF, ON and AUTO; the checksums of the three must agree."""
+    from mathpub_amd import wf
+    engines = {m: wf.Engine(prove=getattr(wf, "PROVE_" + m.upper())) for m in ("off", "on", "auto")}
+    rows, tot = [], {m: 0.0 for m in engines}
+    for h in (10 ** 13, 10 ** 15, 10 ** 17, 10 ** 18, (1 << 64) - (1 << 33)):
+        for L in (10 ** 5, 10 ** 6, 10 ** 7, 10 ** 8, 10 ** 9):
+            row = {"height": h, "length": L}
+            cks = set()
+            for m, e in engines.items():
+                e.scan(h, h + L)
+                best = None
+                for _ in range(max(args.steps, 1)):
+                    r = e.scan(h, h + L)
+                    t = e.last_timing()["sieve_ms"]
+                    best = t if best is None else min(best, t)
+                cks.add((r["primes"], r["cks"]))
+                row[m + "_ms"] = round(best, 3)
+                tot[m] += best
+            assert len(cks) == 1, row
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+    for e in engines.values():
+        e.close()
+    return _engine_line("sieve-stage time of one range per call, full sieve over auto prove mode", tot["off"] / tot["auto"],
+                        "x", tot["auto"], args,
+                        {"workload": "one range per call, lengths 1e5..1e9 at heights 1e13..2^64-2^33",
+                         "total_ms": {m: round(v, 3) for m, v in tot.items()}, "rows": rows},
+                        {"dtype": "u64", "data": "synthetic (the ranges are the input)"})
+
+
+def bench_stats(args):
+    """wf_sieve_stats over [0, 1e12): twin pairs, largest gap, A158939 run lengths (the sieve
+    exposed to the other prime consumers), checked against the reference's data."""
+    from mathpub_amd import wf
+    hi = 10 ** 12
+    eng = wf.Engine()
+    for _ in range(args.warmup):
+        eng.sieve_stats(0, 10 ** 10)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = eng.sieve_stats(0, hi)
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    eng.close()
+    ok = st["twins"] == 1870585220 and st["primes"] == 37607912018 and st["first"].get(13) == 968068681511
+    return _engine_line("numbers sieved/s with twin, gap and run statistics", hi / (ms / 1e3), "numbers/s", ms, args,
+                        {"workload": "[0, 1e12)", "primes": st["primes"], "twins": st["twins"],
+                         "max_gap": st["max_gap"], "max_gap_p": st["max_gap_p"],
+                         "first": {str(k): v for k, v in st["first"].items()},
+                         "matches_reference_data": ok},
+                        {"dtype": "u32", "data": "synthetic (the range is the input)"})
+
+
 def c5_queries(n, seed=0x243F6A8885A308D3 ^ 5):
     """n range queries: lengths log-uniform in [1e5, 1e10], starts uniform in [1e9, 1e18]."""
     import random
@@ -286,7 +341,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
     ap.add_argument("--span", type=float, default=SPAN, help="numbers per step (default 1e11)")
     ap.add_argument("--mode", choices=["both", "plain", "batched"], default="both")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "oeis", "base3"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "oeis", "base3", "prove", "stats"], default="c2",
                     help="c2 = the single range (default, the headline line); c3 = 4096 batched ranges; "
                          "c4 = a slice of the top-of-word range with strong-pseudoprime flags; c5 = serving; "
                          "oeis / base3 = the reference's published speed workloads (17 bases at 2e14, base 3 at 1.2e15)")
@@ -302,7 +357,7 @@ def main():
     args = ap.parse_args()
     if args.workload != "c2" and args.gpus == 1:
         return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "oeis": bench_published,
-                "base3": bench_published}[args.workload](args)
+                "base3": bench_published, "prove": bench_prove, "stats": bench_stats}[args.workload](args)
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
               "(ranges shard by splitting [lo, hi), see DESIGN.md)", file=sys.stderr)
