@@ -163,7 +163,7 @@ __device__ __forceinline__ void walk_multiples(u64 o, u32 q, u64 span, const Whe
 #define WALK_DIV 8        // primes q > span / WALK_DIV walk their multiples instead of 8 class starts
 #endif
 #ifndef FILL_BLOCK
-#define FILL_BLOCK 512    // threads per fill workgroup
+#define FILL_BLOCK 1024   // threads per fill workgroup (16 wavefronts: two workgroups fill a CU)
 #endif
 #ifndef SIEVE_WAVES_EU
 #define SIEVE_WAVES_EU 8  // sieve register budget: 8 wavefronts per SIMD (<= 64 VGPRs, no spills)
@@ -220,13 +220,14 @@ __device__ __forceinline__ bool first_offset(u64 base, u32 q, u64 minv, u64 span
 }
 
 // Every hit of an item's primes in its fill group: hit(segment of the group, byte offset
-// within the run, class), or hit2 for two hits of one class at once (their LDS atomics can
-// then be in flight together).  A lane takes one prime (all 8 classes) or, when the slice
-// is short, one (prime, class) pair; primes with few hits per run walk their multiples.
-template <class F, class F2>
+// within the run, class), or hit4 for four hits of one class at once (their LDS atomics are
+// then in flight together instead of each waiting for the last).  A lane takes one prime
+// (all 8 classes) or, when the slice is short, one (prime, class) pair; primes with few hits
+// per run walk their multiples.
+template <class F, class F4>
 __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __restrict__ fruns,
                                           const u32* __restrict__ sp, const u64* __restrict__ spinv,
-                                          const WheelLds& wl, int sbshift, bool pairs, F hit, F2 hit2) {
+                                          const WheelLds& wl, int sbshift, bool pairs, F hit, F4 hit4) {
   const u32 nwork = pairs ? (it.i1 - it.i0) * 8 : it.i1 - it.i0;
   for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
     const u32 i = it.i0 + (pairs ? w >> 3 : w);
@@ -245,11 +246,17 @@ __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __r
         continue;
       }
       for (int c = c0; c < c1; c++) {
-        // two hits per step: their LDS atomics are in flight together
         u64 j = class_start(o, q, qi, c, wl);
-        for (; j + q < nbytes; j += 2ull * q)
-          hit2(fr.lseg0 + (u32)(j >> sbshift), (u32)j, fr.lseg0 + (u32)((j + q) >> sbshift), (u32)(j + q), (u32)c);
-        if (j < nbytes) hit(fr.lseg0 + (u32)(j >> sbshift), (u32)j, (u32)c);
+        for (; j + 3ull * q < nbytes; j += 4ull * q) {
+          u32 sg[4], jj[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            jj[k] = (u32)(j + (u64)k * q);
+            sg[k] = fr.lseg0 + (jj[k] >> sbshift);
+          }
+          hit4(sg, jj, (u32)c);
+        }
+        for (; j < nbytes; j += q) hit(fr.lseg0 + (u32)(j >> sbshift), (u32)j, (u32)c);
       }
     }
   }
@@ -259,102 +266,75 @@ __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __r
 // fill group: consecutive chunks of segments (of one range, or of several short ranges),
 // RUN_MAX segments at most, sized by the host to about FILL_TARGET expected records.  A
 // prime's table entry is read once per group, and one Barrett reduction per run places it.
-// Walk 1 counts the hits per segment in LDS; one global atomic per segment reserves room
-// in its bucket; walk 2 stages the records in LDS grouped by segment; the workgroup then
-// copies each group to its bucket with consecutive lanes on consecutive addresses.
-// Records that do not fit the stage go straight to the bucket.
+// One walk over the hits: each segment of the group owns C = FILL_STAGE / nseg (a power of
+// two, about twice the expected count) stage slots in LDS, taken by an LDS atomic; a record
+// beyond them (rare) reserves its bucket slot with a global atomic and is written at once.
+// Then one global atomic per segment reserves bucket room for its staged records, and the
+// workgroup copies them out with consecutive lanes on consecutive addresses.
 __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __restrict__ items,
                                                      const FillRun* __restrict__ fruns,
                                                      const u32* __restrict__ sp, const u64* __restrict__ spinv,
                                                      u32* __restrict__ bcnt, u32* __restrict__ bdata,
                                                      u32 bcap, int sbshift, u32* __restrict__ flags) {
-  __shared__ u32 cur[RUN_MAX];    // hits per segment, then the staging cursor
-  __shared__ u32 dloc[RUN_MAX];   // bucket slot of a segment's record = dloc + stage slot
+  __shared__ u32 cur[RUN_MAX];    // records per segment (stage slots taken), then the staged count
+  __shared__ u32 dloc[RUN_MAX];   // bucket slot of a segment's first staged record
   __shared__ u32 gseg[RUN_MAX];   // group segment -> segment of the pass
   __shared__ u32 stage[FILL_STAGE];
-  __shared__ u32 wtot[FILL_BLOCK / 64];
-  __shared__ u32 total;
   __shared__ WheelLds wl;
   if (SIEVE_PRIO) __builtin_amdgcn_s_setprio(SIEVE_PRIO);
   load_wheel(wl);
   const FillDesc it = items[blockIdx.x];
   const u32 smask = (1u << sbshift) - 1;
+  const int cshift = 31 - __builtin_clz(FILL_STAGE / it.nseg);   // it.nseg <= RUN_MAX < FILL_STAGE
+  const u32 C = 1u << cshift;
   for (u32 k = threadIdx.x; k < it.nseg; k += blockDim.x) cur[k] = 0;
   for (u32 r = 0; r < it.nfr; r++) {
     const FillRun fr = fruns[it.fr0 + r];
     for (u32 k = threadIdx.x; k < fr.nseg; k += blockDim.x) gseg[fr.lseg0 + k] = fr.seg0 + k;
   }
   __syncthreads();
+  // a record to its stage slot, or past the segment's room straight to its bucket
+  auto place = [&](u32 sg, u32 slot, u32 rec) {
+    if (slot < C) {
+      stage[(sg << cshift) + slot] = rec;
+    } else {
+      const u32 g = gseg[sg];
+      const u32 pos = atomicAdd(&bcnt[g], 1u);
+      if (pos < bcap) bdata[(u64)g * bcap + pos] = rec;
+      else atomicOr(flags, 1u);
+    }
+  };
   // few primes with many hits each: one lane per (prime, class); else one lane per prime
   const bool pairs = it.i1 - it.i0 < blockDim.x;
-  // walk 1: hits per segment of the group
-  fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32, u32) { atomicAdd(&cur[sg], 1u); },
-            [&](u32 sa, u32, u32 sb, u32, u32) {
-              atomicAdd(&cur[sa], 1u);
-              atomicAdd(&cur[sb], 1u);
-            });
+  fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32 j, u32 c) {
+    place(sg, atomicAdd(&cur[sg], 1u), ((j & smask) << 3) | c);
+  }, [&](const u32 (&sg)[4], const u32 (&jj)[4], u32 c) {
+    u32 slot[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) slot[k] = atomicAdd(&cur[sg[k]], 1u);
+#pragma unroll
+    for (int k = 0; k < 4; k++) place(sg[k], slot[k], ((jj[k] & smask) << 3) | c);
+  });
   __syncthreads();
-  // exclusive prefix sums of the counts (each thread scans a few consecutive segments, the
-  // wavefront and workgroup totals give its start) and one bucket reservation per segment
-  {
-    const u32 per = (it.nseg + blockDim.x - 1) / blockDim.x;
-    const u32 k0 = min(threadIdx.x * per, it.nseg), k1 = min(k0 + per, it.nseg);
-    u32 local = 0;
-    for (u32 k = k0; k < k1; k++) local += cur[k];
-    const u32 incl = wave_incl_scan(local);
-    if ((threadIdx.x & 63) == 63) wtot[threadIdx.x >> 6] = incl;
-    __syncthreads();
-    u32 acc = incl - local;
-    for (u32 w = 0; w < (threadIdx.x >> 6); w++) acc += wtot[w];
-    for (u32 k = k0; k < k1; k++) {
-      const u32 h = cur[k];
-      u32 b = 0;
-      if (h) {
-        b = atomicAdd(&bcnt[gseg[k]], h);
-        if (b + h > bcap) atomicOr(flags, 1u);
-      }
-      dloc[k] = b - acc;   // mod 2^32
-      cur[k] = acc;
-      acc += h;
+  // one bucket reservation per segment for its staged records
+  for (u32 k = threadIdx.x; k < it.nseg; k += blockDim.x) {
+    const u32 n = min(cur[k], C);
+    u32 b = 0;
+    if (n) {
+      b = atomicAdd(&bcnt[gseg[k]], n);
+      if (b + n > bcap) atomicOr(flags, 1u);
     }
-    if (threadIdx.x == blockDim.x - 1) total = acc;
+    dloc[k] = b;
+    cur[k] = n;
   }
   __syncthreads();
-  // walk 2: each record to its stage slot, grouped by segment.  Almost always the item's
-  // records fit the stage (the host sizes items to half of it) and no per-record check is
-  // needed; otherwise the records beyond the stage go straight to their buckets.
-  if (total <= FILL_STAGE) {
-    fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32 j, u32 c) {
-      stage[atomicAdd(&cur[sg], 1u)] = (sg << 20) | ((j & smask) << 3) | c;
-    }, [&](u32 sa, u32 ja, u32 sb, u32 jb, u32 c) {
-      const u32 ka = atomicAdd(&cur[sa], 1u);
-      const u32 kb = atomicAdd(&cur[sb], 1u);
-      stage[ka] = (sa << 20) | ((ja & smask) << 3) | c;
-      stage[kb] = (sb << 20) | ((jb & smask) << 3) | c;
-    });
-  } else {
-    auto put = [&](u32 sg, u32 j, u32 c) {
-      const u32 rec = ((j & smask) << 3) | c;
-      const u32 slot = atomicAdd(&cur[sg], 1u);
-      if (slot < FILL_STAGE) {
-        stage[slot] = (sg << 20) | rec;
-      } else {
-        const u32 pos = dloc[sg] + slot;
-        if (pos < bcap) bdata[(u64)gseg[sg] * bcap + pos] = rec;
-      }
-    };
-    fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, put, [&](u32 sa, u32 ja, u32 sb, u32 jb, u32 c) {
-      put(sa, ja, c);
-      put(sb, jb, c);
-    });
-  }
-  __syncthreads();
-  const u32 nst = min(total, (u32)FILL_STAGE);
-  for (u32 k = threadIdx.x; k < nst; k += blockDim.x) {
-    const u32 v = stage[k];
-    const u32 sg = v >> 20;
-    const u32 pos = dloc[sg] + k;
-    if (pos < bcap) bdata[(u64)gseg[sg] * bcap + pos] = v & 0xFFFFFu;
+  const u32 nslot = it.nseg << cshift;
+  for (u32 x = threadIdx.x; x < nslot; x += blockDim.x) {
+    const u32 k = x >> cshift, i = x & (C - 1);
+    if (i < cur[k]) {
+      const u32 pos = dloc[k] + i;
+      if (pos < bcap) bdata[(u64)gseg[k] * bcap + pos] = stage[x];
+    }
   }
 }
 
