@@ -1495,12 +1495,13 @@ struct PassPlan {
 };
 
 // Prove-mode cost model, in nanoseconds of the whole device, measured on MI355X: placing
-// one sieving prime in one piece (k_bucket_fill: 8 ps at 1e17 .. 2^64), and BPSW per
+// one sieving prime in one piece (k_bucket_fill: 3 to 5 ps from 1e17 to 2^64), BPSW per
 // survivor (base-2 strong test, plus the strong Lucas test for the half that are prime:
-// 0.26 ns), doubled because the bucket fill overlaps the tests of the batched path and
-// BPSW, all VALU work, does not.
-#define PROVE_T_PLACE 0.008
-#define PROVE_T_SURV 0.52
+// 0.24 ns), doubled because the bucket fill overlaps the tests of the batched path and
+// BPSW, all VALU work, does not, and the prove kernels' own start (0.08 ms).
+#define PROVE_T_PLACE 0.0035
+#define PROVE_T_SURV 0.48
+#define PROVE_T_FIXED 80000.0
 
 // Whether piece [lo, hi) should run in prove mode: its sieving primes from bucket_min up
 // would each be placed at least once, against BPSW for its survivors.
@@ -1511,7 +1512,7 @@ static bool prove_piece(const wf_engine* e, u64 lo, u64 hi) {
                                         (u32)std::min<u64>(isqrt_u64(top), 0xFFFFFFFFull)) - e->h_sp.begin());
   if (nq <= e->q1idx) return false;   // nothing to leave out
   if (e->cfg.prove == WF_PROVE_ON) return true;
-  return span_d(lo, hi) * e->surv_frac * PROVE_T_SURV < (double)(nq - e->q1idx) * PROVE_T_PLACE;
+  return span_d(lo, hi) * e->surv_frac * PROVE_T_SURV + PROVE_T_FIXED < (double)(nq - e->q1idx) * PROVE_T_PLACE;
 }
 
 // Entries per segment list: the primes of a segment, or in prove mode its survivors (their
