@@ -102,6 +102,17 @@ __device__ __forceinline__ void f64_mul_pow2(double& a, double& b, int d, const 
   b = __builtin_fma(-kc, m.p, b1);
 }
 
+// The same for p >= 2^32, one reduction shorter: the carry |ka| <= 0.59 * 2^31 + 1 < 0.3 p,
+// so b' = (b 2^d - kb p) + ka stays within 0.8 p, inside the squaring's bound |b| <= 1.9 p.
+__device__ __forceinline__ void f64_mul_pow2_big(double& a, double& b, int d, const F64Mod& m) {
+  const double A = __builtin_ldexp(a, d);
+  const double ka = round_mul(A, m.ip);
+  a = __builtin_fma(-ka, m.p, A);
+  const double B = __builtin_ldexp(b, d);
+  const double kb = round_mul(B, m.ip);
+  b = __builtin_fma(-kb, m.p, B) + ka;
+}
+
 // a in [0, p), then b in [0, p): the canonical digits of r = a + b p.
 __device__ __forceinline__ void f64_normalise(double& a, double& b, const F64Mod& m) {
   if (a < 0.0) { a += m.p; b -= 1.0; }
@@ -135,7 +146,7 @@ __device__ __forceinline__ void fp64_pow2(u64 p, u64 e, int ebits, double& ra, d
 
 // Two independent ladders per lane (two primes): the two instruction streams interleave,
 // which hides FP64 latency without more wavefronts.
-template <int W>
+template <int W, bool BIG = false>
 __device__ __forceinline__ void fp64_pow2x2(u64 p0, u64 e0, u64 p1, u64 e1, int ebits, double& a0, double& b0,
                                             double& a1, double& b1) {
   F64Mod m0, m1;
@@ -158,8 +169,13 @@ __device__ __forceinline__ void fp64_pow2x2(u64 p0, u64 e0, u64 p1, u64 e1, int 
       f64_sqr(a0, b0, m0);
       f64_sqr(a1, b1, m1);
     }
-    f64_mul_pow2(a0, b0, (int)((e0 >> sh) & ((1u << W) - 1)), m0);
-    f64_mul_pow2(a1, b1, (int)((e1 >> sh) & ((1u << W) - 1)), m1);
+    if (BIG) {   // both p >= 2^32
+      f64_mul_pow2_big(a0, b0, (int)((e0 >> sh) & ((1u << W) - 1)), m0);
+      f64_mul_pow2_big(a1, b1, (int)((e1 >> sh) & ((1u << W) - 1)), m1);
+    } else {
+      f64_mul_pow2(a0, b0, (int)((e0 >> sh) & ((1u << W) - 1)), m0);
+      f64_mul_pow2(a1, b1, (int)((e1 >> sh) & ((1u << W) - 1)), m1);
+    }
   }
   f64_normalise(a0, b0, m0);
   f64_normalise(a1, b1, m1);
@@ -182,9 +198,23 @@ __device__ __forceinline__ void f64_mul_small(double& a, double& b, double c, co
   a = a1;
 }
 
+// The same for p >= 2^32, one reduction shorter: |k1| <= 0.59 c + 1 < 0.6 p, so
+// b' = (b c - k2 p + l2) + k1 stays within 1.1 p, inside the squaring's bound |b| <= 1.9 p.
+__device__ __forceinline__ void f64_mul_small_big(double& a, double& b, double c, const F64Mod& m) {
+  const double h1 = a * c;
+  const double l1 = __builtin_fma(a, c, -h1);
+  const double k1 = round_mul(h1, m.ip);
+  const double a1 = __builtin_fma(-k1, m.p, h1) + l1;
+  const double h2 = b * c;
+  const double l2 = __builtin_fma(b, c, -h2);
+  const double k2 = round_mul(h2, m.ip);
+  b = (__builtin_fma(-k2, m.p, h2) + l2) + k1;
+  a = a1;
+}
+
 // Two ladders for a base b with window W: pw[d] = b^d < 2^32 for d < 2^W.  x = b^(leading
-// digit), then per window W squarings and one multiplication by b^d.
-template <int W>
+// digit), then per window W squarings and one multiplication by b^d (BIG: both p >= 2^32).
+template <int W, bool BIG = false>
 __device__ __forceinline__ void fp64_powc_x2(u64 p0, u64 e0, u64 p1, u64 e1, int ebits, const u32* pw,
                                              double& a0, double& b0, double& a1, double& b1) {
   F64Mod m0, m1;
@@ -206,8 +236,13 @@ __device__ __forceinline__ void fp64_powc_x2(u64 p0, u64 e0, u64 p1, u64 e1, int
       f64_sqr(a0, b0, m0);
       f64_sqr(a1, b1, m1);
     }
-    f64_mul_small(a0, b0, (double)pw[(int)((e0 >> sh) & ((1u << W) - 1))], m0);
-    f64_mul_small(a1, b1, (double)pw[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
+    if (BIG) {
+      f64_mul_small_big(a0, b0, (double)pw[(int)((e0 >> sh) & ((1u << W) - 1))], m0);
+      f64_mul_small_big(a1, b1, (double)pw[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
+    } else {
+      f64_mul_small(a0, b0, (double)pw[(int)((e0 >> sh) & ((1u << W) - 1))], m0);
+      f64_mul_small(a1, b1, (double)pw[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
+    }
   }
   f64_normalise(a0, b0, m0);
   f64_normalise(a1, b1, m1);

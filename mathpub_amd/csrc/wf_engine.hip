@@ -805,14 +805,27 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
     const u64 etop = d.base + d.hi_off - 2;
     const int eb = etop ? 64 - __builtin_clzll(etop) : 1;
     if (ARITH == A_F64) {
-      // two primes per lane: i and i + blockDim.x
+      // two primes per lane: i and i + blockDim.x.  A segment at or above 2^32 uses the
+      // shorter window step (idle lanes then take the prime 2^32 + 15).
+      const bool big = d.base >= (1ull << 32);
+      const u64 idle_f = big ? 4294967311ull : idle;
       for (u32 base = beg; base < end; base += 2 * blockDim.x) {
         const u32 i0 = base + threadIdx.x, i1 = i0 + blockDim.x;
         const bool act0 = i0 < end, act1 = i1 < end;
-        const u64 p0 = act0 ? d.base + po[i0] : idle, p1 = act1 ? d.base + po[i1] : idle;
+        const u64 p0 = act0 ? d.base + po[i0] : idle_f, p1 = act1 ? d.base + po[i1] : idle_f;
         double a0, b0, a1, b1;
-        if (b == 2) {
+        if (b == 2 && big) {
+          fp64_pow2x2<5, true>(p0, p0 - 1, p1, p1 - 1, eb, a0, b0, a1, b1);
+        } else if (b == 2) {
           fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, eb, a0, b0, a1, b1);
+        } else if (big) {
+          switch (w) {
+            case 5: fp64_powc_x2<5, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 4: fp64_powc_x2<4, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 3: fp64_powc_x2<3, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 2: fp64_powc_x2<2, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            default: fp64_powc_x2<1, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+          }
         } else {
           switch (w) {
             case 5: fp64_powc_x2<5>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;

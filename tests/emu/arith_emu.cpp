@@ -19,9 +19,10 @@ void emu_fp64(uint64_t p, uint64_t* hi, uint64_t* lo) {
 
 // two ladders at once (the kernel's form), with a wavefront-wide exponent length ebits
 // that may exceed either prime's own
-void emu_fp64x2(uint64_t p0, uint64_t p1, int ebits, uint64_t* r) {
+void emu_fp64x2(uint64_t p0, uint64_t p1, int ebits, uint64_t* r, int big) {
   double a0, b0, a1, b1;
-  fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, ebits, a0, b0, a1, b1);
+  if (big) fp64_pow2x2<5, true>(p0, p0 - 1, p1, p1 - 1, ebits, a0, b0, a1, b1);
+  else fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, ebits, a0, b0, a1, b1);
   const unsigned __int128 r0 = (unsigned __int128)(uint64_t)b0 * p0 + (uint64_t)a0;
   const unsigned __int128 r1 = (unsigned __int128)(uint64_t)b1 * p1 + (uint64_t)a1;
   r[0] = (uint64_t)(r0 >> 64); r[1] = (uint64_t)r0; r[2] = (uint64_t)(r1 >> 64); r[3] = (uint64_t)r1;
@@ -56,17 +57,24 @@ static int base_table(uint32_t b, uint32_t* pw) {
   return w;
 }
 
-void emu_fp64c(uint32_t base, uint64_t p0, uint64_t p1, int ebits, uint64_t* r) {
+void emu_fp64c(uint32_t base, uint64_t p0, uint64_t p1, int ebits, uint64_t* r, int big) {
   uint32_t pw[32];
   const int w = base_table(base, pw);
   double a0, b0, a1, b1;
-  switch (w) {
-    case 5: fp64_powc_x2<5>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
-    case 4: fp64_powc_x2<4>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
-    case 3: fp64_powc_x2<3>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
-    case 2: fp64_powc_x2<2>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
-    default: fp64_powc_x2<1>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;
+#define WF_POWC(B)                                                                              \
+  switch (w) {                                                                                  \
+    case 5: fp64_powc_x2<5, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
+    case 4: fp64_powc_x2<4, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
+    case 3: fp64_powc_x2<3, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
+    case 2: fp64_powc_x2<2, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
+    default: fp64_powc_x2<1, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;       \
   }
+  if (big) {
+    WF_POWC(true)
+  } else {
+    WF_POWC(false)
+  }
+#undef WF_POWC
   const unsigned __int128 r0 = (unsigned __int128)(uint64_t)b0 * p0 + (uint64_t)a0;
   const unsigned __int128 r1 = (unsigned __int128)(uint64_t)b1 * p1 + (uint64_t)a1;
   r[0] = (uint64_t)(r0 >> 64); r[1] = (uint64_t)r0; r[2] = (uint64_t)(r1 >> 64); r[3] = (uint64_t)r1;

@@ -26,8 +26,8 @@ def emu():
     L.emu_sprp2.argtypes = [ctypes.c_uint64]
     L.emu_psp_possible.argtypes = [ctypes.c_uint64]
     L.emu_m28.argtypes = [ctypes.c_uint64, ctypes.c_int, P, P]
-    L.emu_fp64x2.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
-    L.emu_fp64c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
+    L.emu_fp64x2.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P, ctypes.c_int]
+    L.emu_fp64c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P, ctypes.c_int]
     L.emu_m28c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, P, P]
     return L
 
@@ -72,7 +72,23 @@ def test_fp64_two_ladders(emu):
             p1 = 3
         eb = max((p0 - 1).bit_length(), (p1 - 1).bit_length())
         for e in (eb, min(50, eb + rng.randint(0, 7))):
-            emu.emu_fp64x2(p0, p1, e, out)
+            emu.emu_fp64x2(p0, p1, e, out, 0)
+            assert (out[0] << 64 | out[1]) == pow(2, p0 - 1, p0 * p0), (p0, p1, e)
+            assert (out[2] << 64 | out[3]) == pow(2, p1 - 1, p1 * p1), (p0, p1, e)
+
+
+def test_fp64_two_ladders_big(emu):
+    """The shorter window step for p >= 2^32 (segments at or above 2^32): pairs of any sizes
+    from 2^32 + 15 to 2^50, the wavefront's exponent length or more."""
+    ps = [4294967311, (1 << 32) + 61, (1 << 33) + 17, (1 << 50) - 27, (1 << 49) + 1, 10 ** 15 + 37] + \
+        [p for p in samples(33, 50, 12000, 17)]
+    rng = random.Random(13)
+    out = (ctypes.c_uint64 * 4)()
+    for p0 in ps:
+        p1 = rng.choice(ps)
+        eb = max((p0 - 1).bit_length(), (p1 - 1).bit_length())
+        for e in (eb, min(50, eb + rng.randint(0, 7))):
+            emu.emu_fp64x2(p0, p1, e, out, 1)
             assert (out[0] << 64 | out[1]) == pow(2, p0 - 1, p0 * p0), (p0, p1, e)
             assert (out[2] << 64 | out[3]) == pow(2, p1 - 1, p1 * p1), (p0, p1, e)
 
@@ -197,7 +213,25 @@ def test_fp64_any_base(emu):
                 p1 = 3
             eb = max((p0 - 1).bit_length(), (p1 - 1).bit_length())
             eb = min(50, eb + rng.randint(0, 3))
-            emu.emu_fp64c(b, p0, p1, eb, out)
+            emu.emu_fp64c(b, p0, p1, eb, out, 0)
+            assert (out[0] << 64 | out[1]) == pow(b, p0 - 1, p0 * p0), (b, p0)
+            assert (out[2] << 64 | out[3]) == pow(b, p1 - 1, p1 * p1), (b, p1)
+            if p0 >= 1 << 32 and p1 >= 1 << 32:           # the shorter window step
+                emu.emu_fp64c(b, p0, p1, eb, out, 1)
+                assert (out[0] << 64 | out[1]) == pow(b, p0 - 1, p0 * p0), (b, p0)
+                assert (out[2] << 64 | out[3]) == pow(b, p1 - 1, p1 * p1), (b, p1)
+
+
+def test_fp64_any_base_big(emu):
+    """The shorter window step of any base for p >= 2^32, from 2^32 + 15 to 2^50."""
+    rng = random.Random(29)
+    ps = [4294967311, (1 << 32) + 61, (1 << 50) - 27] + samples(33, 50, 600, 31)
+    out = (ctypes.c_uint64 * 4)()
+    for b in BASES:
+        for p0 in ps:
+            p1 = rng.choice(ps)
+            eb = min(50, max((p0 - 1).bit_length(), (p1 - 1).bit_length()) + rng.randint(0, 3))
+            emu.emu_fp64c(b, p0, p1, eb, out, 1)
             assert (out[0] << 64 | out[1]) == pow(b, p0 - 1, p0 * p0), (b, p0)
             assert (out[2] << 64 | out[3]) == pow(b, p1 - 1, p1 * p1), (b, p1)
 
