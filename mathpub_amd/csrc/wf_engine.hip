@@ -548,16 +548,15 @@ __global__ void __launch_bounds__(256) k_prove(const SegDesc* __restrict__ segs,
     const u64 qn = sp[d.nq];
     const u64 lim = qn * qn;                        // q_next < 2^32
     u32 nq = 0;
-    for (u32 j0 = t0; j0 < t1; j0 += 256) {
-      const u32 i = j0 + threadIdx.x;
-      u32 x = 0;
-      bool keep = false;
-      if (i < t1) {
-        x = o[i];
-        const u64 n = d.base + x;
-        keep = n < lim || sprp2_64(n);
-      }
-      nq += block_compact(keep, x, queue + nq, wtot);
+    for (u32 j0 = t0; j0 < t1; j0 += 512) {
+      // two survivors per lane (two test chains), compacted in index order
+      const u32 ia = j0 + threadIdx.x, ib = ia + 256;
+      const u32 xa = ia < t1 ? o[ia] : 0, xb = ib < t1 ? o[ib] : 0;
+      const u64 na = d.base + xa, nb = d.base + xb;
+      bool ra, rb;
+      sprp2_64x2(ia < t1 && na >= lim ? na : 3, ib < t1 && nb >= lim ? nb : 3, ra, rb);
+      nq += block_compact(ia < t1 && (na < lim || ra), xa, queue + nq, wtot);
+      nq += block_compact(ib < t1 && (nb < lim || rb), xb, queue + nq, wtot);
     }
     u32 wp = 0;
     for (u32 k0 = 0; k0 < nq; k0 += 256) {
@@ -877,10 +876,12 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
 // Base-2 strong pseudoprimes: every odd composite n of the segments (wheel composites
 // from the sieve bitmap, and the odd multiples of 3 and 5) that passes the small-factor
 // condition psp_possible is queued in LDS; the workgroup then runs the exact SPRP-2 test
-// on the queue with every lane busy, and the n that pass are appended to out[] and counted
-// per range.
-#define PSP_TILE 4096
+// on the queue with every lane busy, two candidates per lane (two independent chains), and
+// the n that pass are appended to out[] and counted per range.  A 16 KiB queue and at most
+// 85 VGPRs give 6 wavefronts per SIMD.
+#define PSP_TILE 2048
 
+__attribute__((amdgpu_waves_per_eu(6)))
 __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u32 nseg,
                                              const u32* __restrict__ bitmap, u32 bmw, u32 tps,
                                              u64* __restrict__ out, u64 cap, unsigned long long* __restrict__ count,
@@ -922,14 +923,18 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
     }
     __syncthreads();
     const u32 m = qn;
-    for (u32 j = threadIdx.x; j < m; j += blockDim.x) {
-      const u64 n = queue[j];
-      if (sprp2_64(n)) {
-        const unsigned long long kk = atomicAdd(count, 1ull);
-        if (kk < cap) out[kk] = n;
-        atomicAdd(&acc[d.rid].npsp, 1ull);
-        if (slprp_64(n)) atomicAdd(&acc[d.rid].nbpsw, 1ull);   // a BPSW pseudoprime: none below 2^64
-      }
+    auto record = [&](u64 n) {
+      const unsigned long long kk = atomicAdd(count, 1ull);
+      if (kk < cap) out[kk] = n;
+      atomicAdd(&acc[d.rid].npsp, 1ull);
+    };
+    for (u32 j = threadIdx.x; j < m; j += 2 * blockDim.x) {   // two candidates per lane
+      const u32 j1 = j + blockDim.x;
+      const u64 n0 = queue[j], n1 = j1 < m ? queue[j1] : 3;
+      bool r0, r1;
+      sprp2_64x2(n0, n1, r0, r1);
+      if (r0) record(n0);
+      if (r1 && j1 < m) record(n1);
     }
     __syncthreads();   // the queue is refilled by the next tile
   }
@@ -1024,6 +1029,16 @@ __global__ void __launch_bounds__(256) k_stats(const SegDesc* __restrict__ segs,
 __global__ void k_bpsw(const u64* __restrict__ n_in, unsigned char* __restrict__ out, u64 n) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (unsigned char)bpsw_flags(n_in[i]);
+}
+
+// The strong Lucas test of every base-2 strong pseudoprime listed by k_psp (one range per
+// pass with the list): the ones that pass are BPSW pseudoprimes, of which none exist below
+// 2^64.  A kernel of its own keeps the Lucas code's registers out of k_psp.
+__global__ void k_psp_lucas(const u64* __restrict__ list, const unsigned long long* __restrict__ count, u64 cap,
+                            DevAcc* __restrict__ acc) {
+  const u64 n = min((u64)*count, cap);
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+    if (slprp_64(list[i])) atomicAdd(&acc[0].nbpsw, 1ull);
 }
 
 // Zero n words (a kernel, so that the batched path can record it into a HIP graph).
@@ -1869,7 +1884,7 @@ static bool cmp_dev_event(const DevEvent& a, const DevEvent& b) {
 // Scan n ranges; per-range totals into res[0..n), events into ev.
 static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_result* res, wf_event* ev,
                        u64 ev_cap, bool batched, bool want_psp = false, u64* psp = nullptr, u64 psp_cap = 0) {
-  if (!e || !r || !res || n == 0) return WF_E_ARG;
+  if (!e || !r || !res || n == 0 || (want_psp && n != 1)) return WF_E_ARG;
   // the 32-bit limb ladders implement base 2 only
   if (e->cfg.arith == 1 && (e->nb != 1 || e->bases[0] != 2)) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
@@ -1941,6 +1956,11 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   } else {
     for (auto& ps : passes)
       if ((rc = run_pieces(e, ps.first, ps.second, to, want_psp)) != WF_OK) return rc;
+  }
+  // the strong Lucas test of the pseudoprime list (one range: want_psp comes from wf_scan)
+  if (want_psp) {
+    hipLaunchKernelGGL(k_psp_lucas, dim3(64), dim3(256), 0, e->st, e->d_psp, e->d_ctr + 2, e->psp_cap, e->d_acc);
+    HIPCHK(hipGetLastError());
   }
   // results
   std::vector<DevAcc> acc(n);
