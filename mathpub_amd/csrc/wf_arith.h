@@ -829,33 +829,6 @@ __device__ __forceinline__ u64 mont64_one(u64 n) {
   return one;
 }
 
-// Two base-2 strong tests at once (odd n0, n1 >= 3), two independent chains per lane to
-// hide the multiply latency.  One left-to-right pass over the bits of n - 1 from the top bit
-// of the longer one (a chain whose n - 1 is shorter starts at 1, which its leading zero bits
-// keep at 1), squaring and doubling in Montgomery form.  With n - 1 = d 2^s, after bit s the
-// chain holds 2^d (a pass if it is +-1), after bit i < s it holds 2^(d 2^(s - i)) (a pass if
-// it is -1, for i >= 1); bit 0 is not needed.
-__device__ __forceinline__ void sprp2_64x2(u64 n0, u64 n1, bool& r0, bool& r1) {
-  const u64 ni0 = 0 - inv64(n0), ni1 = 0 - inv64(n1);
-  const u64 one0 = mont64_one(n0), one1 = mont64_one(n1);
-  const u64 mone0 = n0 - one0, mone1 = n1 - one1;
-  const u64 e0 = n0 - 1, e1 = n1 - 1;
-  const int s0 = __builtin_ctzll(e0), s1 = __builtin_ctzll(e1);
-  u64 x0 = one0, x1 = one1;
-  bool p0 = false, p1 = false;
-  for (int i = 63 - __builtin_clzll(e0 | e1); i >= 1; i--) {
-    x0 = mont64_sqr(x0, n0, ni0);
-    x1 = mont64_sqr(x1, n1, ni1);
-    const u64 y0 = mont64_dbl(x0, n0), y1 = mont64_dbl(x1, n1);
-    x0 = (e0 >> i) & 1 ? y0 : x0;
-    x1 = (e1 >> i) & 1 ? y1 : x1;
-    p0 = p0 || (i == s0 && x0 == one0) || (i <= s0 && x0 == mone0);
-    p1 = p1 || (i == s1 && x1 == one1) || (i <= s1 && x1 == mone1);
-  }
-  r0 = p0;
-  r1 = p1;
-}
-
 // true iff n is a strong Lucas probable prime with Selfridge's parameters (odd n >= 3)
 __device__ __forceinline__ bool slprp_64(u64 n) {
   if (is_square64(n)) return false;                    // no D would have (D/n) = -1

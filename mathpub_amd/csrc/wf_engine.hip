@@ -548,15 +548,16 @@ __global__ void __launch_bounds__(256) k_prove(const SegDesc* __restrict__ segs,
     const u64 qn = sp[d.nq];
     const u64 lim = qn * qn;                        // q_next < 2^32
     u32 nq = 0;
-    for (u32 j0 = t0; j0 < t1; j0 += 512) {
-      // two survivors per lane (two test chains), compacted in index order
-      const u32 ia = j0 + threadIdx.x, ib = ia + 256;
-      const u32 xa = ia < t1 ? o[ia] : 0, xb = ib < t1 ? o[ib] : 0;
-      const u64 na = d.base + xa, nb = d.base + xb;
-      bool ra, rb;
-      sprp2_64x2(ia < t1 && na >= lim ? na : 3, ib < t1 && nb >= lim ? nb : 3, ra, rb);
-      nq += block_compact(ia < t1 && (na < lim || ra), xa, queue + nq, wtot);
-      nq += block_compact(ib < t1 && (nb < lim || rb), xb, queue + nq, wtot);
+    for (u32 j0 = t0; j0 < t1; j0 += 256) {
+      const u32 i = j0 + threadIdx.x;
+      u32 x = 0;
+      bool keep = false;
+      if (i < t1) {
+        x = o[i];
+        const u64 n = d.base + x;
+        keep = n < lim || sprp2_64(n);
+      }
+      nq += block_compact(keep, x, queue + nq, wtot);
     }
     u32 wp = 0;
     for (u32 k0 = 0; k0 < nq; k0 += 256) {
@@ -876,12 +877,16 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
 // Base-2 strong pseudoprimes: every odd composite n of the segments (wheel composites
 // from the sieve bitmap, and the odd multiples of 3 and 5) that passes the small-factor
 // condition psp_possible is queued in LDS; the workgroup then runs the exact SPRP-2 test
-// on the queue with every lane busy, two candidates per lane (two independent chains), and
-// the n that pass are appended to out[] and counted per range.  A 16 KiB queue and at most
-// 85 VGPRs give 6 wavefronts per SIMD.
+// on the queue with every lane busy, and the n that pass are appended to out[] and counted
+// per range.
+#ifndef PSP_TILE
 #define PSP_TILE 2048
+#endif
+#ifndef PSP_WAVES_EU
+#define PSP_WAVES_EU 6
+#endif
 
-__attribute__((amdgpu_waves_per_eu(6)))
+__attribute__((amdgpu_waves_per_eu(PSP_WAVES_EU)))
 __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u32 nseg,
                                              const u32* __restrict__ bitmap, u32 bmw, u32 tps,
                                              u64* __restrict__ out, u64 cap, unsigned long long* __restrict__ count,
@@ -928,13 +933,9 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
       if (kk < cap) out[kk] = n;
       atomicAdd(&acc[d.rid].npsp, 1ull);
     };
-    for (u32 j = threadIdx.x; j < m; j += 2 * blockDim.x) {   // two candidates per lane
-      const u32 j1 = j + blockDim.x;
-      const u64 n0 = queue[j], n1 = j1 < m ? queue[j1] : 3;
-      bool r0, r1;
-      sprp2_64x2(n0, n1, r0, r1);
-      if (r0) record(n0);
-      if (r1 && j1 < m) record(n1);
+    for (u32 j = threadIdx.x; j < m; j += blockDim.x) {
+      const u64 n = queue[j];
+      if (sprp2_64(n)) record(n);
     }
     __syncthreads();   // the queue is refilled by the next tile
   }

@@ -100,11 +100,6 @@ int emu_psp_possible(uint64_t n) { return psp_possible(n) ? 1 : 0; }
 
 int emu_slprp(uint64_t n) { return slprp_64(n) ? 1 : 0; }
 
-int emu_sprp2x2(uint64_t n0, uint64_t n1) {
-  bool r0, r1;
-  sprp2_64x2(n0, n1, r0, r1);
-  return (r0 ? 1 : 0) | (r1 ? 2 : 0);
-}
 
 uint32_t emu_bpsw(uint64_t n) { return bpsw_flags(n); }
 

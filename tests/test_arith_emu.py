@@ -136,26 +136,6 @@ def test_sprp2_64(emu):
         assert bool(emu.emu_sprp2(n)) == strong2(n), n
 
 
-def test_sprp2_two_chains(emu):
-    """sprp2_64x2 (two tests per lane, one pass over the bits of n - 1) against the single
-    test and the definition: every pair of odd n below 600, the strong pseudoprimes, and random
-    pairs of equal and of different lengths up to 2^64."""
-    emu.emu_sprp2x2.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
-    small = list(range(3, 600, 2))
-    for a in small:
-        for b in small[::7]:
-            assert emu.emu_sprp2x2(a, b) == strong2(a) | (strong2(b) << 1), (a, b)
-    rng = random.Random(43)
-    ns = [2047, 3277, 4033, 4681, 8321, 3825123056546413051, 18446744066047760377, (1 << 64) - 59, (1 << 64) - 1,
-          (1 << 63) + 1, 3, 5, 7, 9]
-    ns += [rng.getrandbits(64) | 1 for _ in range(2000)] + [rng.getrandbits(rng.randint(2, 63)) | 1 for _ in range(2000)]
-    ns += [q for q in (rng.randrange(1 << 60, 1 << 64) | 1 for _ in range(3000)) if _isprime(q)]
-    ns = [n for n in ns if n >= 3]
-    for i in range(len(ns)):
-        a, b = ns[i], ns[(i * 7 + 3) % len(ns)]
-        assert emu.emu_sprp2x2(a, b) == strong2(a) | (strong2(b) << 1), (a, b)
-
-
 def test_radix28_montgomery(emu):
     ps = samples(47, 64, 8000, 99) + [(1 << 47) + 1, (1 << 50) - 27, (1 << 63) - 25, (1 << 63) + 29,
                                       (1 << 64) - 59, (1 << 64) - 1, (1 << 64) - 3, 10 ** 18 + 9]
