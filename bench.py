@@ -110,7 +110,7 @@ def bench_c3(args):
     ranges = c3_ranges()
     eng = wf.Engine(arith=args.arith, prove=_prove(args))
     t = time.perf_counter()
-    tuned = None if args.no_tune else eng.autotune(ranges[0][0], ranges[0][1])
+    tuned = None if args.no_tune else eng.autotune_batch(ranges)
     print("autotune %s (%.1f s)" % (tuned, time.perf_counter() - t), file=sys.stderr, flush=True)
     for i in range(args.warmup):
         t = time.perf_counter()

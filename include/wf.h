@@ -172,6 +172,9 @@ int  wf_serve_stop(wf_engine *e);
  * to the fastest and returns it (*best_ms = its sieve milliseconds per 1e9 numbers).
  * No counterpart in the reference (its chunk and thread counts are set by hand). */
 int  wf_autotune(wf_engine *e, uint64_t lo, uint64_t hi, wf_config *best, double *best_ms);
+/* The same for a batch of ranges (wf_scan_batch's workload): the sample is the ranges in
+ * order, up to 3e9 numbers, sieved as one packed pass. */
+int  wf_autotune_batch(wf_engine *e, const wf_range *r, uint32_t n, wf_config *best, double *best_ms);
 
 /* Timings of the last wf_scan / wf_scan_batch: device milliseconds of the sieve and
  * the test kernels (HIP events on the engine's stream), primes tested, device passes, the

@@ -769,7 +769,11 @@ __device__ __forceinline__ void residue(u64 p, int eb, u32 b, int w, const u32* 
 // Work item = one tile of up to 1024 primes of one segment, for one base.  Workgroups take
 // items in turn and keep their sums until the (range, base) changes (uniform across the
 // workgroup).  The bases' power tables are copied to LDS once per workgroup.
+#ifndef M28_WAVES_EU
+#define M28_WAVES_EU 6   // the radix-2^28 test kernel: 6 wavefronts per SIMD (<= 80 VGPRs)
+#endif
 template <int ARITH>
+__attribute__((amdgpu_waves_per_eu(ARITH == A_M28 ? M28_WAVES_EU : 1)))
 __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, u32 nseg,
                                               const u32* __restrict__ off, const u32* __restrict__ cnt, u32 pcap,
                                               u32 tps, TestOut o) {
@@ -2220,16 +2224,11 @@ static void apply_config(wf_engine* e, const wf_config& c) {
 // Autotuner: time the sieve stage (bucket fill + segment sieve + extraction) of a sample
 // of [lo, hi) for every segment size 16-128 KiB, sieve workgroup of 4-16 wavefronts and
 // bucket threshold 2^15-2^20, keep the fastest.  The test kernel does not depend on them.
-extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double* best_ms) {
-  if (!e || (hi != 0 && hi <= lo)) return WF_E_ARG;
-  HIPCHK(hipSetDevice(e->dev));
-  const u64 top = hi ? hi - 1 : 0xFFFFFFFFFFFFFFFFull;
+// Time the sieve stage of the given pieces for every configuration, keep the fastest.
+static int autotune_pieces(wf_engine* e, const std::vector<Piece>& pieces, double numbers, wf_config* best,
+                           double* best_ms) {
   int rc;
-  if ((rc = ensure_sieving_primes(e, std::max<u64>(isqrt_u64(top) + 1, 64))) != WF_OK) return rc;
   if ((rc = grow(e->d_acc, e->acc_cap, 1)) != WF_OK) return rc;
-  // sample: up to 3e9 numbers from the start of the range
-  const double whole = span_d(lo, hi);
-  const u64 sample = (u64)std::min(whole, 3.0e9);
   const wf_config orig = e->cfg;
   wf_config bestc = orig;
   double bestt = 1e300;
@@ -2244,7 +2243,6 @@ extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double
         c.sieve_block = bl;
         c.bucket_min = bm;
         apply_config(e, c);
-        std::vector<Piece> pieces = {{lo, lo + sample, 0, A_F64}};
         PassPlan P;
         plan_pass(e, pieces, P);
         if ((rc = size_set(e->bs[0], P)) != WF_OK) { apply_config(e, orig); return rc; }
@@ -2268,8 +2266,39 @@ extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double
       }
   apply_config(e, bestc);
   if (best) *best = e->cfg;
-  if (best_ms) *best_ms = bestt / ((double)sample / 1e9);   // ms per 1e9 numbers
+  if (best_ms) *best_ms = bestt / (numbers / 1e9);   // ms per 1e9 numbers
   return WF_OK;
+}
+
+extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double* best_ms) {
+  if (!e || (hi != 0 && hi <= lo)) return WF_E_ARG;
+  HIPCHK(hipSetDevice(e->dev));
+  const u64 top = hi ? hi - 1 : 0xFFFFFFFFFFFFFFFFull;
+  int rc;
+  if ((rc = ensure_sieving_primes(e, std::max<u64>(isqrt_u64(top) + 1, 64))) != WF_OK) return rc;
+  // sample: up to 3e9 numbers from the start of the range
+  const u64 sample = (u64)std::min(span_d(lo, hi), 3.0e9);
+  return autotune_pieces(e, {{lo, lo + sample, 0, A_F64}}, (double)sample, best, best_ms);
+}
+
+extern "C" int wf_autotune_batch(wf_engine* e, const wf_range* r, u32 n, wf_config* best, double* best_ms) {
+  if (!e || !r || n == 0) return WF_E_ARG;
+  HIPCHK(hipSetDevice(e->dev));
+  // sample: the ranges in order, up to 3e9 numbers (the last one cut), packed as one pass
+  std::vector<Piece> pieces;
+  double total = 0;
+  u64 maxtop = 0;
+  for (u32 i = 0; i < n && total < 3.0e9; i++) {
+    if (r[i].hi != 0 && r[i].hi <= r[i].lo) return WF_E_ARG;
+    const double len = std::min(span_d(r[i].lo, r[i].hi), 3.0e9 - total);
+    const u64 hi = r[i].lo + (u64)len;
+    pieces.push_back({r[i].lo, hi, 0, A_F64});
+    maxtop = std::max<u64>(maxtop, hi ? hi - 1 : 0xFFFFFFFFFFFFFFFFull);
+    total += len;
+  }
+  int rc;
+  if ((rc = ensure_sieving_primes(e, std::max<u64>(isqrt_u64(maxtop) + 1, 64))) != WF_OK) return rc;
+  return autotune_pieces(e, pieces, total, best, best_ms);
 }
 
 extern "C" int wf_last_timing(wf_engine* e, wf_timing* t) {

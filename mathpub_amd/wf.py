@@ -84,13 +84,15 @@ def load():
     L.wf_sieve_stats.argtypes = [ctypes.c_void_p, u64, u64, ctypes.POINTER(SieveStats)]
     L.wf_last_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(Timing)]
     L.wf_autotune.argtypes = [ctypes.c_void_p, u64, u64, ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_double)]
+    L.wf_autotune_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Range), u32, ctypes.POINTER(Config),
+                                    ctypes.POINTER(ctypes.c_double)]
     L.wf_serve_start.argtypes = [ctypes.c_void_p, u32, u64]
     L.wf_serve_submit.argtypes = [ctypes.c_void_p, u64, u64, P64]
     L.wf_serve_poll.argtypes = [ctypes.c_void_p, u64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_int)]
     L.wf_serve_stop.argtypes = [ctypes.c_void_p]
     for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_quotients_base",
               "wf_primes", "wf_bpsw", "wf_sieve_stats",
-              "wf_last_timing", "wf_autotune", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
+              "wf_last_timing", "wf_autotune", "wf_autotune_batch", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
         getattr(L, f).restype = ctypes.c_int
     _lib = L
     return L
@@ -219,6 +221,15 @@ class Engine:
         """Pick segment size, sieve workgroup and bucket threshold for [lo, hi) (wf_autotune)."""
         c, ms = Config(), ctypes.c_double()
         _check(self._L.wf_autotune(self._h, lo, _hi(hi), ctypes.byref(c), ctypes.byref(ms)), "wf_autotune")
+        return {"seg_bytes": c.seg_bytes, "sieve_block": c.sieve_block, "bucket_min": c.bucket_min,
+                "sieve_ms_per_1e9": ms.value}
+
+    def autotune_batch(self, ranges):
+        """The same for a batch of ranges (wf_autotune_batch): the first ranges, up to 3e9 numbers."""
+        n = len(ranges)
+        rr = (Range * n)(*[Range(lo, _hi(hi)) for lo, hi in ranges])
+        c, ms = Config(), ctypes.c_double()
+        _check(self._L.wf_autotune_batch(self._h, rr, n, ctypes.byref(c), ctypes.byref(ms)), "wf_autotune_batch")
         return {"seg_bytes": c.seg_bytes, "sieve_block": c.sieve_block, "bucket_min": c.bucket_min,
                 "sieve_ms_per_1e9": ms.value}
 

@@ -131,6 +131,27 @@ def test_autotune_keeps_results(eng):
         assert (x["primes"], x["cks"], x["sumq"]) == (before["primes"], before["cks"], before["sumq"])
 
 
+def test_autotune_batch_keeps_results():
+    """wf_autotune_batch picks a configuration from a sample of the batch; every range's
+    result is the same before and after."""
+    rng = random.Random(71)
+    ranges = []
+    for _ in range(40):
+        lo = rng.randrange(10 ** 12, 10 ** 18)
+        ranges.append((lo, lo + int(10 ** rng.uniform(5, 8))))
+    e = wf.Engine()
+    try:
+        before, ev0 = e.scan_batch(ranges, near_T=100)
+        cfg = e.autotune_batch(ranges)
+        say("autotune_batch", cfg)
+        assert cfg["seg_bytes"] in (16384, 32768, 65536, 131072)
+        after, ev1 = e.scan_batch(ranges, near_T=100)
+        assert [(x["primes"], x["cks"], x["sumq"]) for x in before] == [(x["primes"], x["cks"], x["sumq"]) for x in after]
+        assert ev0 == ev1
+    finally:
+        e.close()
+
+
 @pytest.mark.parametrize("seg,block,bmin", [(16384, 256, 512), (65536, 1024, 262144), (131072, 1024, 524288),
                                             (131072, 512, 32768)])
 def test_segment_configs(seg, block, bmin):
