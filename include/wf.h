@@ -53,7 +53,9 @@ enum {
     WF_E_NOMEM = -3,        /* device or host allocation failed */
     WF_E_UNSUPPORTED = -4,  /* a base other than 2, or a feature not built */
     WF_E_OVERFLOW = -5,     /* an internal buffer bound was exceeded (bug guard) */
-    WF_E_BUSY = -6,         /* serving: no free slot / HBM budget */
+    WF_E_BUSY = -6,         /* serving: no free slot / HBM budget; or a call that needs the
+                               engine (scans, quotients, BPSW, statistics, autotuning,
+                               timings, bases) while it serves, until wf_serve_stop */
 };
 
 /* Event types, the reference's result lines (wieferich.c:1009, :1060, :1071). */

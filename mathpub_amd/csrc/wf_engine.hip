@@ -1133,7 +1133,7 @@ extern "C" const char* wf_strerror(int rc) {
     case WF_E_NOMEM: return "out of memory";
     case WF_E_UNSUPPORTED: return "not supported by the GPU engine (base 2 only)";
     case WF_E_OVERFLOW: return "internal buffer bound exceeded";
-    case WF_E_BUSY: return "serving: no free slot";
+    case WF_E_BUSY: return "engine busy serving (no free slot, or wf_serve_stop not called yet)";
     default: return "unknown error";
   }
 }
@@ -1890,6 +1890,7 @@ static bool cmp_dev_event(const DevEvent& a, const DevEvent& b) {
 static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_result* res, wf_event* ev,
                        u64 ev_cap, bool batched, bool want_psp = false, u64* psp = nullptr, u64 psp_cap = 0) {
   if (!e || !r || !res || n == 0 || (want_psp && n != 1)) return WF_E_ARG;
+  if (e->srv) return WF_E_BUSY;   // the server thread owns the engine until wf_serve_stop
   // the 32-bit limb ladders implement base 2 only
   if (e->cfg.arith == 1 && (e->nb != 1 || e->bases[0] != 2)) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
@@ -2054,6 +2055,7 @@ extern "C" int wf_scan_batch(wf_engine* e, const wf_range* r, u32 n, int64_t nea
 
 extern "C" int wf_quotients_base(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
   if (!e || !p || !q || !r_hi || !r_lo || b < 2) return WF_E_ARG;
+  if (e->srv) return WF_E_BUSY;
   if (n == 0) return WF_OK;
   for (u64 i = 0; i < n; i++)
     if (p[i] < 3 || !(p[i] & 1)) return WF_E_ARG;
@@ -2085,6 +2087,7 @@ extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* 
 
 extern "C" int wf_bpsw(wf_engine* e, const u64* n, uint8_t* flags, u64 count) {
   if (!e || (count && (!n || !flags))) return WF_E_ARG;
+  if (e->srv) return WF_E_BUSY;
   if (count == 0) return WF_OK;
   HIPCHK(hipSetDevice(e->dev));
   u64* d = nullptr;
@@ -2105,6 +2108,7 @@ extern "C" int wf_bpsw(wf_engine* e, const u64* n, uint8_t* flags, u64 count) {
 
 extern "C" int wf_sieve_stats(wf_engine* e, u64 lo, u64 hi, wf_stats* out) {
   if (!e || !out || hi <= lo || hi > WF_STATS_MAXHI) return WF_E_ARG;
+  if (e->srv) return WF_E_BUSY;
   HIPCHK(hipSetDevice(e->dev));
   int rc;
   if ((rc = ensure_sieving_primes(e, isqrt_u64(hi - 1 + STATS_OV) + 1)) != WF_OK) return rc;
@@ -2169,6 +2173,7 @@ extern "C" int wf_sieve_stats(wf_engine* e, u64 lo, u64 hi, wf_stats* out) {
 
 extern "C" int wf_primes(wf_engine* e, u64 lo, u64 hi, u64* out, u64 cap, u64* count) {
   if (!e || !count || (cap && !out) || (hi != 0 && hi <= lo)) return WF_E_ARG;
+  if (e->srv) return WF_E_BUSY;
   HIPCHK(hipSetDevice(e->dev));
   const u64 top = hi ? hi - 1 : 0xFFFFFFFFFFFFFFFFull;
   int rc;
@@ -2272,6 +2277,7 @@ static int autotune_pieces(wf_engine* e, const std::vector<Piece>& pieces, doubl
 
 extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double* best_ms) {
   if (!e || (hi != 0 && hi <= lo)) return WF_E_ARG;
+  if (e->srv) return WF_E_BUSY;
   HIPCHK(hipSetDevice(e->dev));
   const u64 top = hi ? hi - 1 : 0xFFFFFFFFFFFFFFFFull;
   int rc;
@@ -2283,6 +2289,7 @@ extern "C" int wf_autotune(wf_engine* e, u64 lo, u64 hi, wf_config* best, double
 
 extern "C" int wf_autotune_batch(wf_engine* e, const wf_range* r, u32 n, wf_config* best, double* best_ms) {
   if (!e || !r || n == 0) return WF_E_ARG;
+  if (e->srv) return WF_E_BUSY;
   HIPCHK(hipSetDevice(e->dev));
   // sample: the ranges in order, up to 3e9 numbers (the last one cut), packed as one pass
   std::vector<Piece> pieces;
@@ -2303,6 +2310,7 @@ extern "C" int wf_autotune_batch(wf_engine* e, const wf_range* r, u32 n, wf_conf
 
 extern "C" int wf_last_timing(wf_engine* e, wf_timing* t) {
   if (!e || !t) return WF_E_ARG;
+  if (e->srv) return WF_E_BUSY;
   *t = e->last;
   return WF_OK;
 }

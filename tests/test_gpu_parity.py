@@ -184,6 +184,13 @@ def test_serving_same_alone_and_in_mix():
         want = [alone.scan(lo, hi) for lo, hi in ranges]
     with wf.Engine() as eng:
         eng.serve_start(slots=16, hbm_budget=8 << 30)
+        # while it serves, the engine belongs to the server thread: other calls are refused
+        for call in (lambda: eng.scan(10 ** 12, 10 ** 12 + 1000), lambda: eng.quotients([1093]),
+                     lambda: eng.bpsw([7]), lambda: eng.primes(0, 100), lambda: eng.sieve_stats(0, 100),
+                     lambda: eng.set_bases([2, 3])):
+            with pytest.raises(wf.WFError) as ex:
+                call()
+            assert ex.value.rc == -6
         pending = list(enumerate(ranges))
         live, got = {}, {}
         t0 = time.time()
