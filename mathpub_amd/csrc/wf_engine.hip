@@ -612,7 +612,9 @@ __global__ void __launch_bounds__(PROVE_TILE) k_prove_gather(const SegDesc* __re
 // tests
 // ---------------------------------------------------------------------------------
 
+#ifndef TEST_TILE
 #define TEST_TILE 1024   // primes per test work item (4 rounds of 256 lanes)
+#endif
 
 // arithmetic classes: FP64 p-adic (p < 2^50), 32-bit Montgomery with 2 / 3 / 4 limbs,
 // 4 limbs keeping the carry bit (p >= 2^63), radix-2^28 Montgomery (p >= 2^47)
