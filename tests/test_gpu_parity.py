@@ -76,7 +76,8 @@ def test_golden_windows(eng, w):
 
 
 @pytest.mark.parametrize("lo,hi", [(10 ** 15, 10 ** 15 + 2 * 10 ** 8), (10 ** 12 + 12345, 10 ** 12 + 10 ** 8 + 7),
-                                   ((1 << 50) - 10 ** 7, (1 << 50) + 10 ** 7), (3, 10 ** 8)])
+                                   ((1 << 50) - 10 ** 7, (1 << 50) + 10 ** 7), (3, 10 ** 8),
+                                   ((1 << 32) - 3 * 10 ** 6, (1 << 32) + 3 * 10 ** 6)])
 def test_scan_vs_oracle(eng, lo, hi):
     say("scan vs oracle", lo, hi)
     r = eng.scan(lo, hi, near_T=1000)
@@ -270,6 +271,7 @@ def test_known_solutions_all_bases(feng):
 
 
 @pytest.mark.parametrize("lo,hi", [(3, 2 * 10 ** 6), (10 ** 12, 10 ** 12 + 2 * 10 ** 7),
+                                   ((1 << 32) - 2 * 10 ** 6, (1 << 32) + 2 * 10 ** 6),
                                    ((1 << 50) - 5 * 10 ** 6, (1 << 50) + 5 * 10 ** 6),
                                    ((1 << 63) - 5 * 10 ** 6, (1 << 63) + 5 * 10 ** 6), ((1 << 64) - 10 ** 7, 1 << 64)])
 def test_any_bases_vs_oracle(feng, lo, hi):
