@@ -430,8 +430,12 @@ def main():
                     "frac": achieved / (INT32_MUL_PEAK / 1e12), "traffic": None, "kernel": "k_test<A_M4>"}
         else:
             achieved = tests * flops / (test_ms / 1e3) / 1e12
-            roof = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            # compute-bound ("mfma" in the roofline's two-way split): the FP64 peak, 78.6 TFLOP/s,
+            # is MI355X's published FP64 figure; the kernel issues it on the vector pipe, since
+            # every lane has its own modulus and there is no matrix product
+            roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_test<A_F64>",
+                    "compute_pipe": "FP64 VALU (v_fma_f64); no MFMA: one modulus per lane",
                     "valu_fp64_issue_frac": tests * instr / (test_ms / 1e3) / FP64_PEAK_INSTR}
         tj = os.path.join(ROOT, "profiles", "c2_k_test_traffic.json")
         if roof["kernel"] == "k_test<A_F64>" and span == SPAN and os.path.exists(tj):
