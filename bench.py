@@ -426,8 +426,11 @@ def main():
         flops, instr = fp64_flops(pbits)
         if args.arith == 1 or (LO + span) > (1 << 50):
             achieved = tests * wf_products(pbits) / (test_ms / 1e3) / 1e12
-            roof = {"bound": "valu", "achieved": achieved, "peak": INT32_MUL_PEAK / 1e12, "unit": "Tmul32/s",
-                    "frac": achieved / (INT32_MUL_PEAK / 1e12), "traffic": None, "kernel": "k_test<A_M4>"}
+            # compute-bound on the integer multiplier: the peak is the measured issue rate of
+            # v_mad_u64_u32 (no published figure exists for it)
+            roof = {"bound": "mfma", "achieved": achieved, "peak": INT32_MUL_PEAK / 1e12, "unit": "Tmul32/s",
+                    "frac": achieved / (INT32_MUL_PEAK / 1e12), "traffic": None, "kernel": "k_test<A_M4>",
+                    "compute_pipe": "integer VALU (v_mad_u64_u32); no MFMA: one modulus per lane"}
         else:
             achieved = tests * flops / (test_ms / 1e3) / 1e12
             # compute-bound ("mfma" in the roofline's two-way split): the FP64 peak, 78.6 TFLOP/s,
