@@ -88,11 +88,21 @@ __constant__ unsigned char c_QI[30];
 // c_W30[r] = index of r in R30, -1 if r shares a factor with 30
 __constant__ signed char c_W30[30] = {-1, 0, -1, -1, -1, -1, -1, 1, -1, -1, -1, 2, -1, 3, -1,
                                       -1, -1, 4, -1, 5, -1, -1, -1, 6, -1, -1, -1, -1, -1, 7};
+// Wheel walk tables.  For a prime q of class qi and a multiple n = base + o = q m (base
+// divisible by 30, t = o mod 30): c_WS[qi][t] = (j << 3) | k, where m + j is the first
+// cofactor at or after m that is prime to 30 and R30[k] is its residue.  c_WCLS[qi] holds,
+// in its 4-bit field k, the class (index in R30) of q R30[k] mod 30.
+__constant__ unsigned char c_WS[8 * 30];
+__constant__ u32 c_WCLS[8];
+// gaps between consecutive residues of R30 (from R30[k] to R30[k + 1], 29 to 31), field k
+#define WHEEL_GAPS 0x26424246u
 
-// The two wheel tables, copied to LDS by every workgroup that looks them up per lane
+// The wheel tables, copied to LDS by every workgroup that looks them up per lane
 // (per-lane lookups in constant memory are vector memory loads with cache latency).
 struct WheelLds {
   unsigned char m30[8 * 30 * 8];
+  unsigned char ws[8 * 30];
+  u32 wcls[8];
   unsigned char qi[32];
   signed char w30[32];
   unsigned char r30[8];
@@ -100,11 +110,15 @@ struct WheelLds {
 
 __device__ __forceinline__ void load_wheel(WheelLds& w) {
   for (u32 k = threadIdx.x; k < 8 * 30 * 8; k += blockDim.x) w.m30[k] = c_M30[k];
+  for (u32 k = threadIdx.x; k < 8 * 30; k += blockDim.x) w.ws[k] = c_WS[k];
   if (threadIdx.x < 30) {
     w.qi[threadIdx.x] = c_QI[threadIdx.x];
     w.w30[threadIdx.x] = c_W30[threadIdx.x];
   }
-  if (threadIdx.x < 8) w.r30[threadIdx.x] = (unsigned char)c_R30[threadIdx.x];
+  if (threadIdx.x < 8) {
+    w.r30[threadIdx.x] = (unsigned char)c_R30[threadIdx.x];
+    w.wcls[threadIdx.x] = c_WCLS[threadIdx.x];
+  }
 }
 
 // Byte index j (relative to the segment base) of the first mark of class c for a prime q
@@ -120,16 +134,31 @@ __device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c, const Wh
   return (u64)((u32)(v >> 1) * 0xEEEEEEEFu);
 }
 
-// The multiples x = o, o + q, ... below span (offsets from a base divisible by 30, span
-// < 2^32) of a prime q: f(byte, class) for each one on the wheel.  For q > span / 8 (a few
-// multiples at most) this costs less than finding the eight class starts.
-template <class F>
-__device__ __forceinline__ void walk_multiples(u64 o, u32 q, u64 span, const WheelLds& w, F f) {
-  for (u64 x = o; x < span; x += q) {
-    const u32 xb = (u32)x;
-    const u32 byte = xb / 30u;
-    const int c = w.w30[xb - byte * 30u];
-    if (c >= 0) f(byte, c);
+__device__ __forceinline__ u32 rotr(u32 x, u32 r) { return (x >> r) | (x << ((32u - r) & 31u)); }   // r < 32
+
+// Step of a wheel walk: gap * q, gap <= 6.  The 32-bit walk keeps q < 2^24 (one 24-bit
+// multiply-add); the 64-bit walk takes any q.
+__device__ __forceinline__ u32 wheel_step(u32 gap, u32 q) { return gap * (q & 0xFFFFFFu); }
+__device__ __forceinline__ u64 wheel_step(u64 gap, u32 q) { return gap * q; }
+
+// The multiples of a prime q > 5 on the wheel at offsets [o, span) from a base divisible by
+// 30, in increasing order: f(byte, class) for each.  With base + o = q m, the multiple
+// q (m + j) is on the wheel exactly when m + j is prime to 30, so the walk starts at the
+// first such cofactor (c_WS) and then steps by q times the gaps between consecutive wheel
+// residues; the class of each multiple rotates through c_WCLS[qi] in step.  Offsets are
+// T: u32 needs q < 2^24 and span + 6 q < 2^32, u64 any q and span < 2^32.  With a few
+// multiples per class this costs less than finding the eight class starts.
+template <class T, class F>
+__device__ __forceinline__ void wheel_walk(T o, u32 q, int qi, T span, const WheelLds& w, F f) {
+  const u32 st = w.ws[qi * 30 + (u32)o % 30u];   // o < span < 2^32
+  const u32 r = (st & 7u) * 4u;                   // field 0 = the first cofactor's residue
+  u32 gaps = rotr(WHEEL_GAPS, r), cls = rotr(w.wcls[qi], r);
+  T x = o + wheel_step((T)(st >> 3), q);
+  while (x < span) {
+    f((u32)x / 30u, (int)(cls & 7u));
+    x += wheel_step((T)(gaps & 15u), q);
+    gaps = rotr(gaps, 4);
+    cls = rotr(cls, 4);
   }
 }
 
@@ -160,7 +189,10 @@ __device__ __forceinline__ void walk_multiples(u64 o, u32 q, u64 span, const Whe
 #define SIEVE_PRIO 3
 #endif
 #ifndef WALK_DIV
-#define WALK_DIV 8        // primes q > span / WALK_DIV walk their multiples instead of 8 class starts
+#define WALK_DIV 8        // fill: primes q > span / WALK_DIV walk their multiples instead of 8 class starts
+#endif
+#ifndef WHEEL_DIV
+#define WHEEL_DIV 4       // sieve: primes q > seg_bytes / WHEEL_DIV walk their multiples
 #endif
 #ifndef FILL_BLOCK
 #define FILL_BLOCK 1024   // threads per fill workgroup (16 wavefronts: two workgroups fill a CU)
@@ -242,7 +274,7 @@ __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __r
       u64 o;
       if (!first_offset(fr.base, q, minv, span, o)) continue;
       if (!pairs && (u64)WALK_DIV * q > span) {
-        walk_multiples(o, q, span, wl, [&](u32 j, int c) { hit(fr.lseg0 + (j >> sbshift), j, (u32)c); });
+        wheel_walk<u64>(o, q, qi, span, wl, [&](u32 j, int c) { hit(fr.lseg0 + (j >> sbshift), j, (u32)c); });
         continue;
       }
       for (int c = c0; c < c1; c++) {
@@ -421,11 +453,13 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
         }
         u64 o;
         if (!first_offset(d.base, q, minv, span, o)) continue;
-        if ((u64)WALK_DIV * q > span) {
-          walk_multiples(o, q, span, wl, [&](u32 j, int c) { atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c)); });
+        const int qi = wl.qi[q % 30];
+        if ((u64)WHEEL_DIV * q > d.nbytes) {     // fewer than WHEEL_DIV marks per class
+          auto mark = [&](u32 j, int c) { atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c)); };
+          if (q < (1u << 24)) wheel_walk<u32>((u32)o, q, qi, (u32)span, wl, mark);
+          else wheel_walk<u64>(o, q, qi, span, wl, mark);
           continue;
         }
-        const int qi = wl.qi[q % 30];
 #pragma unroll 1
         for (int c = 0; c < 8; c++) {
           for (u32 j = (u32)class_start(o, q, qi, c, wl); j < d.nbytes; j += q)
@@ -1307,8 +1341,27 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
         m30[(k * 30 + o) * 8 + c] = (unsigned char)m;
       }
   }
+  // wheel walk: for q = R30[k] (mod 30) and t = q m (mod 30), the first j >= 0 with m + j
+  // prime to 30 and the index of (m + j) mod 30 in R30; the class of q R30[i] for each i
+  unsigned char ws[8 * 30];
+  u32 wcls[8];
+  for (int k = 0; k < 8; k++) {
+    const u32 q = R30H[k];
+    u32 qinv = 1;
+    while (qinv * q % 30 != 1) qinv++;
+    for (u32 t = 0; t < 30; t++) {
+      const u32 m = t * qinv % 30;
+      u32 j = 0;
+      while ((m + j) % 2 == 0 || (m + j) % 3 == 0 || (m + j) % 5 == 0) j++;
+      ws[k * 30 + t] = (unsigned char)((j << 3) | qi[(m + j) % 30]);
+    }
+    wcls[k] = 0;
+    for (int i = 0; i < 8; i++) wcls[k] |= (u32)qi[q * R30H[i] % 30] << (4 * i);
+  }
   if (hipMemcpyToSymbol(HIP_SYMBOL(c_M30), m30, sizeof m30) != hipSuccess ||
-      hipMemcpyToSymbol(HIP_SYMBOL(c_QI), qi, sizeof qi) != hipSuccess || put_base(0, 2) != WF_OK) {
+      hipMemcpyToSymbol(HIP_SYMBOL(c_QI), qi, sizeof qi) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_WS), ws, sizeof ws) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_WCLS), wcls, sizeof wcls) != hipSuccess || put_base(0, 2) != WF_OK) {
     wf_close(e);
     return WF_E_DEVICE;
   }
