@@ -33,14 +33,20 @@ def wf_products(pbits):
     return (pbits - 1) * (L * (L + 1) // 2 + L + L * L) + (L + L * L)
 
 
-def fp64_flops(pbits, W=5):
-    """FP64 flops of one test in the p-adic kernel (FMA = 2): ceil(bits/W) windows of
-    W squarings (18 flops, 13 instructions) and one 2^d step (15 flops, 12 instructions)."""
-    nw = -(-pbits // W)
+def fp64_flops(ebits, big=True, W=5):
+    """FP64 flops (FMA = 2) and instructions of one base-2 test in the p-adic kernel
+    (fp64_pow2x2), exponent length ebits.  A squaring is 2 mul + 7 FMA + 4 add (20 flops, 13
+    instructions).  For p >= 2^32 (big): the six leading bits at once (5 flops, 3), then
+    ebits - 6 squarings and ceil((ebits - 6) / W) steps x 2^d of 11 flops (9 instructions).
+    Below 2^32: the leading W-bit digit, then ceil(ebits / W) - 1 windows of W squarings and
+    a 16-flop (12-instruction) step.  Plus the set-up, one division 1/p (12 flops, 10)."""
+    if big:
+        sq = ebits - 6
+        steps = -(-sq // W)
+        return sq * 20 + steps * 11 + 5 + 12, sq * 13 + steps * 9 + 3 + 10
+    nw = -(-ebits // W)
     sq = (nw - 1) * W
-    flops = sq * 18 + nw * 15 + 12          # + set-up (1/p, 2/p, p/2) and normalisation
-    instr = sq * 13 + nw * 12 + 10
-    return flops, instr
+    return sq * 20 + (nw - 1) * 16 + 12, sq * 13 + (nw - 1) * 12 + 10
 
 
 def cpu_baseline(target_s=8.0):
@@ -423,7 +429,7 @@ def main():
         sieve_ms = sum(t["sieve_ms"] for t in ptim) / args.steps
         launches = max(1, ptim[-1]["launches"])
         pbits = (LO + span // 2).bit_length()
-        flops, instr = fp64_flops(pbits)
+        flops, instr = fp64_flops(pbits, big=LO >= (1 << 32))
         if args.arith == 1 or (LO + span) > (1 << 50):
             achieved = tests * wf_products(pbits) / (test_ms / 1e3) / 1e12
             # compute-bound on the integer multiplier: the peak is the measured issue rate of

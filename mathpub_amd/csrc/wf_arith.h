@@ -4,13 +4,13 @@
 // Two exact implementations of r = 2^(p-1) mod p^2 (A001220/wieferich.c:893-949 computes
 // the same residue with 2 x 64-bit Montgomery limbs and a 5-bit window):
 //
-//  * fp64_fermat2: p < 2^50.  The residue is held in p-adic digits x = a + b p with a, b
-//    signed integers stored exactly in doubles.  A squaring needs a^2 mod p, floor(a^2/p)
-//    and 2ab mod p; each product is split exactly by FMA (h = x*y, l = fma(x, y, -h)) and
-//    reduced with a rounded quotient k = rint(h / p) and an exact FMA remainder.  gfx950
-//    issues FP64 FMA at the full VALU rate, four times the 32-bit integer multiply rate.
+//  * fp64_pow2 / fp64_pow2x2: p < 2^50.  The residue is held in p-adic digits x = a + b p
+//    with a, b signed integers stored exactly in doubles.  A squaring needs a^2 mod p,
+//    floor(a^2/p) and 2ab mod p; each product is split exactly by FMA (h = x*y,
+//    l = fma(x, y, -h)) and reduced with a rounded quotient k = rint(h / p) and an exact FMA
+//    remainder.  gfx950 issues v_fma_f64 at the full VALU rate, faster than v_mad_u64_u32.
 //
-//  * mont_fermat2<L, FULL>: any odd p < 2^64.  L 32-bit limbs, R = 2^(32 L), products
+//  * mont_pow2<L, FULL>: any odd p < 2^64.  L 32-bit limbs, R = 2^(32 L), products
 //    from v_mad_u64_u32 / v_mul_lo_u32 chains (operand scanning), lazy values in [0, 2N)
 //    while N < R/4 and full reduction with the carry bit (FULL) when N may reach 2^128.
 //    The base-2 ladder is left to right: square, then double when the exponent bit is 1.
@@ -69,8 +69,8 @@ struct F64Mod {
 
 // Round-to-nearest integer of x*y + 0 for |x*y| < 2^51: adding 1.5 * 2^52 inside the FMA
 // leaves a sum in [2^52, 2^53), whose unit in the last place is 1, so the single rounding
-// of the FMA is the rounding to an integer.  (v_rndne_f64 issues at half rate on gfx950;
-// FMA and add at full rate.)
+// of the FMA is the rounding to an integer.  Rounding the exact product (rather than
+// rint(fl(x*y)), also two instructions) keeps the quotient error bounds below.
 #define WF_RND_MAGIC 6755399441055744.0
 __device__ __forceinline__ double round_mul(double x, double y) {
   return __builtin_fma(x, y, WF_RND_MAGIC) - WF_RND_MAGIC;
@@ -144,25 +144,52 @@ __device__ __forceinline__ void fp64_pow2(u64 p, u64 e, int ebits, double& ra, d
   rb = b;
 }
 
-// Two independent ladders per lane (two primes): the two instruction streams interleave,
-// which hides FP64 latency without more wavefronts.
+// Two independent ladders per lane (two primes, given as doubles pd = p exactly): the two
+// instruction streams interleave, which hides FP64 latency without more wavefronts.  The
+// digits are left as the last window step leaves them (|a| <= p/2 (1 + 2^-20), |b| <= p),
+// for f64_quotient or f64_normalise.
+//
+// Leading bits.  In general x = 2^d for the leading window digit d.  With BIG (both p >=
+// 2^32, ebits >= 32) the six leading bits t < 64 are taken at once: 2^t = a + k p with
+// k = round(2^t / p) < 2^31 < p/2 and a = 2^t - k p exact, |a| <= p/2; then a first window
+// of (ebits - 6) mod W bits, so that the W-bit windows end at bit 0.  Against the plain
+// leading digit this saves 6 - (ebits mod W or W) squarings: one at 50 bits.
 template <int W, bool BIG = false>
-__device__ __forceinline__ void fp64_pow2x2(u64 p0, u64 e0, u64 p1, u64 e1, int ebits, double& a0, double& b0,
-                                            double& a1, double& b1) {
+__device__ __forceinline__ void fp64_pow2x2(double pd0, u64 e0, double pd1, u64 e1, int ebits, double& a0,
+                                            double& b0, double& a1, double& b1) {
   F64Mod m0, m1;
-  m0.p = (double)p0;
-  m0.ip = 1.0 / m0.p;
-  m1.p = (double)p1;
-  m1.ip = 1.0 / m1.p;
-  const int nw = (ebits + W - 1) / W;
-  int sh = (nw - 1) * W;
-  // the leading digit: x = 2^d directly when that is already below p
-  const int d0 = (int)(e0 >> sh), d1 = (int)(e1 >> sh);
-  a0 = 1.0; b0 = 0.0; a1 = 1.0; b1 = 0.0;
-  if ((1ull << d0) < p0) a0 = __builtin_ldexp(1.0, d0);
-  else f64_mul_pow2(a0, b0, d0, m0);
-  if ((1ull << d1) < p1) a1 = __builtin_ldexp(1.0, d1);
-  else f64_mul_pow2(a1, b1, d1, m1);
+  m0.p = pd0;
+  m0.ip = 1.0 / pd0;
+  m1.p = pd1;
+  m1.ip = 1.0 / pd1;
+  int sh;
+  if (BIG) {
+    sh = ebits - 6;
+    const double x0 = __builtin_ldexp(1.0, (int)(e0 >> sh)), x1 = __builtin_ldexp(1.0, (int)(e1 >> sh));
+    b0 = round_mul(x0, m0.ip);
+    a0 = __builtin_fma(-b0, m0.p, x0);
+    b1 = round_mul(x1, m1.ip);
+    a1 = __builtin_fma(-b1, m1.p, x1);
+    const int r = sh % W;
+    if (r) {
+      sh -= r;
+      for (int s = 0; s < r; s++) {
+        f64_sqr(a0, b0, m0);
+        f64_sqr(a1, b1, m1);
+      }
+      f64_mul_pow2_big(a0, b0, (int)((e0 >> sh) & ((1u << r) - 1)), m0);
+      f64_mul_pow2_big(a1, b1, (int)((e1 >> sh) & ((1u << r) - 1)), m1);
+    }
+  } else {
+    const int nw = (ebits + W - 1) / W;
+    sh = (nw - 1) * W;
+    // the leading digit: x = 2^d directly when that is already below p
+    const int d0 = (int)(e0 >> sh), d1 = (int)(e1 >> sh);
+    a0 = __builtin_ldexp(1.0, d0); b0 = 0.0;
+    a1 = __builtin_ldexp(1.0, d1); b1 = 0.0;
+    if (a0 >= pd0) { a0 = 1.0; f64_mul_pow2(a0, b0, d0, m0); }
+    if (a1 >= pd1) { a1 = 1.0; f64_mul_pow2(a1, b1, d1, m1); }
+  }
   for (sh -= W; sh >= 0; sh -= W) {
 #pragma unroll
     for (int s = 0; s < W; s++) {
@@ -177,8 +204,22 @@ __device__ __forceinline__ void fp64_pow2x2(u64 p0, u64 e0, u64 p1, u64 e1, int 
       f64_mul_pow2(a1, b1, (int)((e1 >> sh) & ((1u << W) - 1)), m1);
     }
   }
-  f64_normalise(a0, b0, m0);
-  f64_normalise(a1, b1, m1);
+}
+
+// The digits of r = a + b p as a ladder leaves them: |a| <= p/2 (1 + 2^-20) after a window
+// step (0 <= a < p if none ran) and |b| <= 1.1 p.  Then r = 1 (mod p), the FLT condition,
+// holds exactly when a == 1, and q = (r - 1) / p = b mod p takes at most two corrections.
+// q < 2^50 is read from the bits of q + 2^52 (exponent fixed, the mantissa is q).  Returns
+// false, leaving q alone, when a != 1.
+__device__ __forceinline__ bool f64_quotient(double a, double b, double pd, u64& q) {
+  if (a != 1.0) return false;
+  double t = b < 0.0 ? b + pd : b;                      // (-0.1 p, 1.1 p]
+  t = t < 0.0 ? t + pd : (t >= pd ? t - pd : t);
+  const double s = t + 4503599627370496.0;
+  u64 bits;
+  __builtin_memcpy(&bits, &s, sizeof bits);
+  q = bits & 0xFFFFFFFFFFFFFull;
+  return true;
 }
 
 // x *= c for an integer 0 < c < 2^32 (any base's window multiplier b^d): a c and b c are
@@ -214,21 +255,22 @@ __device__ __forceinline__ void f64_mul_small_big(double& a, double& b, double c
 
 // Two ladders for a base b with window W: pw[d] = b^d < 2^32 for d < 2^W.  x = b^(leading
 // digit), then per window W squarings and one multiplication by b^d (BIG: both p >= 2^32).
+// The digits are left raw, as fp64_pow2x2 leaves them.
 template <int W, bool BIG = false>
-__device__ __forceinline__ void fp64_powc_x2(u64 p0, u64 e0, u64 p1, u64 e1, int ebits, const u32* pw,
+__device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64 e1, int ebits, const u32* pw,
                                              double& a0, double& b0, double& a1, double& b1) {
   F64Mod m0, m1;
-  m0.p = (double)p0;
-  m0.ip = 1.0 / m0.p;
-  m1.p = (double)p1;
-  m1.ip = 1.0 / m1.p;
+  m0.p = pd0;
+  m0.ip = 1.0 / pd0;
+  m1.p = pd1;
+  m1.ip = 1.0 / pd1;
   const int nw = (ebits + W - 1) / W;
   int sh = (nw - 1) * W;
   const u32 c0 = pw[(int)(e0 >> sh)], c1 = pw[(int)(e1 >> sh)];
   a0 = 1.0; b0 = 0.0; a1 = 1.0; b1 = 0.0;
-  if ((u64)c0 < p0) a0 = (double)c0;
+  if ((double)c0 < pd0) a0 = (double)c0;
   else f64_mul_small(a0, b0, (double)c0, m0);
-  if ((u64)c1 < p1) a1 = (double)c1;
+  if ((double)c1 < pd1) a1 = (double)c1;
   else f64_mul_small(a1, b1, (double)c1, m1);
   for (sh -= W; sh >= 0; sh -= W) {
 #pragma unroll
@@ -244,8 +286,6 @@ __device__ __forceinline__ void fp64_powc_x2(u64 p0, u64 e0, u64 p1, u64 e1, int
       f64_mul_small(a1, b1, (double)pw[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
     }
   }
-  f64_normalise(a0, b0, m0);
-  f64_normalise(a1, b1, m1);
 }
 
 // ---------------------------------------------------------------------------------

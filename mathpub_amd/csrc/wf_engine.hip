@@ -685,9 +685,16 @@ __device__ __forceinline__ void padic_to_r(u64 p, double a, double b, u64& rhi, 
   rhi = hi + (rlo < lo);
 }
 
+// |A| <= T for the signed quotient A = q (q <= p/2) or q - p (q > p/2), wieferich.c's
+// near-hit test: |A| = min(q, p - q) for every p and q < p.
+__device__ __forceinline__ bool near_hit(u64 p, u64 q, i64 T) {
+  const u64 A = q < p - q ? q : p - q;
+  return A <= (u64)T;
+}
+
 // Account one tested (prime, base) pair exactly as finish_lanes (wieferich.c:1051-1075)
 // and add the lane's terms to its running sums.  r = r_hi:r_lo, q = (r - 1) p^-1 mod 2^64.
-struct LaneSums { u64 tests, flt, near, cks, sumq; };
+struct LaneSums { u32 tests, flt, near; u64 cks, sumq; };   // counts per lane and launch fit 32 bits
 
 __device__ __forceinline__ void account(const TestOut& o, u32 rid, u32 j, u64 p, u64 rhi, u64 rlo, LaneSums& s) {
   const u64 pinv = inv64(p);
@@ -702,37 +709,36 @@ __device__ __forceinline__ void account(const TestOut& o, u32 rid, u32 j, u64 p,
   s.sumq += q;
   if (q == 0) {
     push_event(o, WF_EV_FOUND, rid, j, p, q, rhi, rlo);
-  } else if (o.near_T > 0) {
-    const i64 A = q <= p / 2 ? (i64)q : -(i64)(p - q);
-    if ((A < 0 ? -A : A) <= o.near_T) {
-      s.near++;
-      push_event(o, WF_EV_NEAR, rid, j, p, q, rhi, rlo);
-    }
+  } else if (o.near_T > 0 && near_hit(p, q, o.near_T)) {
+    s.near++;
+    push_event(o, WF_EV_NEAR, rid, j, p, q, rhi, rlo);
   }
 }
 
-// The same from the p-adic digits of r = a + b p: for prime p (not dividing the base), a = 1
-// and q = b exactly, so p^-1 and r are only formed for an event or a failed FLT check.
-__device__ __forceinline__ void account_padic(const TestOut& o, u32 rid, u32 j, u64 p, double a, double b,
-                                              LaneSums& s) {
-  if (a == 1.0) {
-    const u64 q = (u64)b;                      // < p: the FLT check holds
+// The same from the ladder's p-adic digits of r = a + b p (pd = p as a double): for prime p
+// not dividing the base, r = 1 (mod p), so a = 1 and q = b mod p (f64_quotient); p^-1 and r
+// are only formed for an event or a failed FLT check.
+__device__ __forceinline__ void account_padic(const TestOut& o, u32 rid, u32 j, u64 p, double pd, double a,
+                                              double b, LaneSums& s) {
+  u64 q;
+  if (f64_quotient(a, b, pd, q)) {             // q < p: the FLT check holds
     s.tests++;
     s.cks += d_mix64(q ^ (p * GOLD));
     s.sumq += q;
     if (q == 0) {
       push_event(o, WF_EV_FOUND, rid, j, p, 0, 0, 1);
-    } else if (o.near_T > 0) {
-      const i64 A = q <= p / 2 ? (i64)q : -(i64)(p - q);
-      if ((A < 0 ? -A : A) <= o.near_T) {
-        s.near++;
-        const u64 lo = q * p, hi = __umul64hi(q, p);
-        const u64 rlo = lo + 1;
-        push_event(o, WF_EV_NEAR, rid, j, p, q, hi + (rlo < lo), rlo);
-      }
+    } else if (o.near_T > 0 && near_hit(p, q, o.near_T)) {
+      s.near++;
+      const u64 lo = q * p, hi = __umul64hi(q, p);
+      const u64 rlo = lo + 1;
+      push_event(o, WF_EV_NEAR, rid, j, p, q, hi + (rlo < lo), rlo);
     }
     return;
   }
+  F64Mod m;
+  m.p = pd;
+  m.ip = 1.0 / pd;
+  f64_normalise(a, b, m);
   u64 rhi, rlo;
   padic_to_r(p, a, b, rhi, rlo);
   account(o, rid, j, p, rhi, rlo, s);
@@ -740,7 +746,7 @@ __device__ __forceinline__ void account_padic(const TestOut& o, u32 rid, u32 j, 
 
 // Wavefront sums of a (range, base)'s lane totals -> one atomic per counter (lane 0).
 __device__ __forceinline__ void flush_sums(const TestOut& o, u32 rid, u32 j, LaneSums& s) {
-  const u64 t = wave_sum(s.tests), f = wave_sum(s.flt), n = wave_sum(s.near);
+  const u64 t = wave_sum((u64)s.tests), f = wave_sum((u64)s.flt), n = wave_sum((u64)s.near);
   const u64 c = wave_sum(s.cks), q = wave_sum(s.sumq);
   if ((threadIdx.x & 63) == 0 && t) {
     DevAcc* a = &o.acc[rid];
@@ -763,18 +769,23 @@ template <int ARITH>
 __device__ __forceinline__ void residue(u64 p, int eb, u32 b, int w, const u32* pw, u64& rhi, u64& rlo) {
   const u64 e = p - 1;
   if (ARITH == A_F64) {
+    const double pd = (double)p;
     double a0, b0, a1, b1;
     if (b == 2) {
-      fp64_pow2x2<5>(p, e, 3, 2, eb, a0, b0, a1, b1);
+      fp64_pow2x2<5>(pd, e, 3.0, 2, eb, a0, b0, a1, b1);
     } else {
       switch (w) {
-        case 5: fp64_powc_x2<5>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
-        case 4: fp64_powc_x2<4>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
-        case 3: fp64_powc_x2<3>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
-        case 2: fp64_powc_x2<2>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
-        default: fp64_powc_x2<1>(p, e, 3, 2, eb, pw, a0, b0, a1, b1); break;
+        case 5: fp64_powc_x2<5>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
+        case 4: fp64_powc_x2<4>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
+        case 3: fp64_powc_x2<3>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
+        case 2: fp64_powc_x2<2>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
+        default: fp64_powc_x2<1>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
       }
     }
+    F64Mod m;
+    m.p = pd;
+    m.ip = 1.0 / pd;
+    f64_normalise(a0, b0, m);
     padic_to_r(p, a0, b0, rhi, rlo);
   } else if (ARITH == A_M28) {
     u32 x[4];
@@ -808,8 +819,11 @@ __device__ __forceinline__ void residue(u64 p, int eb, u32 b, int w, const u32* 
 #ifndef M28_WAVES_EU
 #define M28_WAVES_EU 6   // the radix-2^28 test kernel: 6 wavefronts per SIMD (<= 80 VGPRs)
 #endif
+#ifndef F64_WAVES_EU
+#define F64_WAVES_EU 6   // the FP64 test kernel: 6 wavefronts per SIMD (<= 80 VGPRs)
+#endif
 template <int ARITH>
-__attribute__((amdgpu_waves_per_eu(ARITH == A_M28 ? M28_WAVES_EU : 1)))
+__attribute__((amdgpu_waves_per_eu(ARITH == A_M28 ? M28_WAVES_EU : ARITH == A_F64 ? F64_WAVES_EU : 1)))
 __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, u32 nseg,
                                               const u32* __restrict__ off, const u32* __restrict__ cnt, u32 pcap,
                                               u32 tps, TestOut o) {
@@ -849,34 +863,38 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
       // shorter window step (idle lanes then take the prime 2^32 + 15).
       const bool big = d.base >= (1ull << 32);
       const u64 idle_f = big ? 4294967311ull : idle;
+      const double based = (double)d.base;     // p = base + offset < 2^50: exact as a double
       for (u32 base = beg; base < end; base += 2 * blockDim.x) {
         const u32 i0 = base + threadIdx.x, i1 = i0 + blockDim.x;
         const bool act0 = i0 < end, act1 = i1 < end;
-        const u64 p0 = act0 ? d.base + po[i0] : idle_f, p1 = act1 ? d.base + po[i1] : idle_f;
+        const u32 f0 = act0 ? po[i0] : 0, f1 = act1 ? po[i1] : 0;
+        const u64 p0 = act0 ? d.base + f0 : idle_f, p1 = act1 ? d.base + f1 : idle_f;
+        const double pd0 = act0 ? based + (double)f0 : (double)idle_f;
+        const double pd1 = act1 ? based + (double)f1 : (double)idle_f;
         double a0, b0, a1, b1;
         if (b == 2 && big) {
-          fp64_pow2x2<5, true>(p0, p0 - 1, p1, p1 - 1, eb, a0, b0, a1, b1);
+          fp64_pow2x2<5, true>(pd0, p0 - 1, pd1, p1 - 1, eb, a0, b0, a1, b1);
         } else if (b == 2) {
-          fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, eb, a0, b0, a1, b1);
+          fp64_pow2x2<5>(pd0, p0 - 1, pd1, p1 - 1, eb, a0, b0, a1, b1);
         } else if (big) {
           switch (w) {
-            case 5: fp64_powc_x2<5, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 4: fp64_powc_x2<4, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 3: fp64_powc_x2<3, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 2: fp64_powc_x2<2, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            default: fp64_powc_x2<1, true>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 5: fp64_powc_x2<5, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 4: fp64_powc_x2<4, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 3: fp64_powc_x2<3, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 2: fp64_powc_x2<2, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            default: fp64_powc_x2<1, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
           }
         } else {
           switch (w) {
-            case 5: fp64_powc_x2<5>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 4: fp64_powc_x2<4>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 3: fp64_powc_x2<3>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 2: fp64_powc_x2<2>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            default: fp64_powc_x2<1>(p0, p0 - 1, p1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 5: fp64_powc_x2<5>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 4: fp64_powc_x2<4>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 3: fp64_powc_x2<3>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 2: fp64_powc_x2<2>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            default: fp64_powc_x2<1>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
           }
         }
-        if (act0 && !divides_base(p0, b)) account_padic(o, d.rid, j, p0, a0, b0, s);
-        if (act1 && !divides_base(p1, b)) account_padic(o, d.rid, j, p1, a1, b1, s);
+        if (act0 && !divides_base(p0, b)) account_padic(o, d.rid, j, p0, pd0, a0, b0, s);
+        if (act1 && !divides_base(p1, b)) account_padic(o, d.rid, j, p1, pd1, a1, b1, s);
       }
     } else {
       for (u32 base = beg; base < end; base += blockDim.x) {

@@ -17,15 +17,30 @@ void emu_fp64(uint64_t p, uint64_t* hi, uint64_t* lo) {
   *lo = (uint64_t)r;
 }
 
+// The kernel's finish of one ladder's raw digits: r (normalised, r[0] = high, r[1] = low)
+// and f64_quotient's verdict: r[2] = 1 and r[3] = q when it takes the fast path, else 0, 0.
+static void finish(uint64_t p, double a, double b, uint64_t* r) {
+  uint64_t q = 0;
+  const bool fast = f64_quotient(a, b, (double)p, q);
+  F64Mod m;
+  m.p = (double)p;
+  m.ip = 1.0 / m.p;
+  f64_normalise(a, b, m);
+  const unsigned __int128 x = (unsigned __int128)(uint64_t)b * p + (uint64_t)a;
+  r[0] = (uint64_t)(x >> 64);
+  r[1] = (uint64_t)x;
+  r[2] = fast;
+  r[3] = fast ? q : 0;
+}
+
 // two ladders at once (the kernel's form), with a wavefront-wide exponent length ebits
-// that may exceed either prime's own
+// that may exceed either prime's own; r[0..3] for p0 and r[4..7] for p1 as finish() gives
 void emu_fp64x2(uint64_t p0, uint64_t p1, int ebits, uint64_t* r, int big) {
   double a0, b0, a1, b1;
-  if (big) fp64_pow2x2<5, true>(p0, p0 - 1, p1, p1 - 1, ebits, a0, b0, a1, b1);
-  else fp64_pow2x2<5>(p0, p0 - 1, p1, p1 - 1, ebits, a0, b0, a1, b1);
-  const unsigned __int128 r0 = (unsigned __int128)(uint64_t)b0 * p0 + (uint64_t)a0;
-  const unsigned __int128 r1 = (unsigned __int128)(uint64_t)b1 * p1 + (uint64_t)a1;
-  r[0] = (uint64_t)(r0 >> 64); r[1] = (uint64_t)r0; r[2] = (uint64_t)(r1 >> 64); r[3] = (uint64_t)r1;
+  if (big) fp64_pow2x2<5, true>((double)p0, p0 - 1, (double)p1, p1 - 1, ebits, a0, b0, a1, b1);
+  else fp64_pow2x2<5>((double)p0, p0 - 1, (double)p1, p1 - 1, ebits, a0, b0, a1, b1);
+  finish(p0, a0, b0, r);
+  finish(p1, a1, b1, r + 4);
 }
 
 // the same through the L-limb Montgomery ladder; full = carry-keeping variant
@@ -60,14 +75,15 @@ static int base_table(uint32_t b, uint32_t* pw) {
 void emu_fp64c(uint32_t base, uint64_t p0, uint64_t p1, int ebits, uint64_t* r, int big) {
   uint32_t pw[32];
   const int w = base_table(base, pw);
+  const double d0 = (double)p0, d1 = (double)p1;
   double a0, b0, a1, b1;
 #define WF_POWC(B)                                                                              \
   switch (w) {                                                                                  \
-    case 5: fp64_powc_x2<5, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
-    case 4: fp64_powc_x2<4, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
-    case 3: fp64_powc_x2<3, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
-    case 2: fp64_powc_x2<2, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
-    default: fp64_powc_x2<1, B>(p0, p0 - 1, p1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;       \
+    case 5: fp64_powc_x2<5, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
+    case 4: fp64_powc_x2<4, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
+    case 3: fp64_powc_x2<3, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
+    case 2: fp64_powc_x2<2, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
+    default: fp64_powc_x2<1, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;       \
   }
   if (big) {
     WF_POWC(true)
@@ -75,9 +91,8 @@ void emu_fp64c(uint32_t base, uint64_t p0, uint64_t p1, int ebits, uint64_t* r, 
     WF_POWC(false)
   }
 #undef WF_POWC
-  const unsigned __int128 r0 = (unsigned __int128)(uint64_t)b0 * p0 + (uint64_t)a0;
-  const unsigned __int128 r1 = (unsigned __int128)(uint64_t)b1 * p1 + (uint64_t)a1;
-  r[0] = (uint64_t)(r0 >> 64); r[1] = (uint64_t)r0; r[2] = (uint64_t)(r1 >> 64); r[3] = (uint64_t)r1;
+  finish(p0, a0, b0, r);
+  finish(p1, a1, b1, r + 4);
 }
 
 void emu_m28c(uint32_t base, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
