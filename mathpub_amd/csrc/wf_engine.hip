@@ -1148,6 +1148,7 @@ struct wf_engine {
   u64* d_spinv = nullptr;
   u32 tierA = 0, q1idx = 0;
   double surv_frac = 1.0;        // share of numbers with no prime factor below bucket_min
+  size_t fill_hint = 0;          // fill items of the last planned pass (a reserve size)
   // pass descriptors (all passes of a batch) and two sets of pass buffers
   SegDesc* d_segs = nullptr;
   u64 seg_cap = 0;
@@ -1550,6 +1551,12 @@ static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, bool pro
   const u32 SB = e->cfg.seg_bytes;
   const u64 span = 30ull * SB;
   u64 base = lo - lo % 30;
+  // isqrt of the segment's last number and the count of sieving primes up to it: both grow
+  // with the segment, so each advances from the previous segment's value
+  u64 r = 0;
+  u32 nq = 0;
+  bool first = true;
+  const u32 nsp = (u32)e->h_sp.size();
   for (;;) {
     SegDesc d;
     d.base = base;
@@ -1564,9 +1571,17 @@ static void build_segments(const wf_engine* e, u64 lo, u64 hi, u32 rid, bool pro
     if (base == 0 && d.lo_off < 2) d.lo_off = 2;   // 1 is not prime
     d.hi_off = (u32)len;
     const u64 last = base + (len - 1);
-    const u64 r = isqrt_u64(last);
-    d.nq = (u32)(std::upper_bound(e->h_sp.begin(), e->h_sp.end(), (u32)std::min<u64>(r, 0xFFFFFFFFull)) -
+    if (first) {
+      first = false;
+      r = isqrt_u64(last);
+      nq = (u32)(std::upper_bound(e->h_sp.begin(), e->h_sp.end(), (u32)std::min<u64>(r, 0xFFFFFFFFull)) -
                  e->h_sp.begin());
+    } else {
+      while ((unsigned __int128)(r + 1) * (r + 1) <= last) r++;
+      const u32 rq = (u32)std::min<u64>(r, 0xFFFFFFFFull);
+      while (nq < nsp && e->h_sp[nq] <= rq) nq++;
+    }
+    d.nq = nq;
     d.prove = 0;
     if (prove && d.nq > e->q1idx) {   // sieve with the primes below bucket_min, prove the rest
       d.nq = e->q1idx;
@@ -1669,6 +1684,7 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
   P.tps = (P.pcap + TEST_TILE - 1) / TEST_TILE;
   P.bcap = 0;
   P.fills.clear();
+  P.fills.reserve(e->fill_hint);
   if (P.maxnq > e->q1idx) {
     // expected records per segment: sum over large primes and classes of seg_bytes / q; the
     // count is a sum of independent 0/1 (or floor/ceil) terms, so its deviation is ~sqrt(E)
@@ -1733,6 +1749,7 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
       i = gend;
     }
   }
+  e->fill_hint = P.fills.size();
 }
 
 // Prove tiles per segment list.
