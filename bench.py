@@ -470,6 +470,9 @@ def main():
         cfg["batched_over_plain"] = value / pv
     if batched:
         cfg["batched_host_plan_ms_per_step"] = sum(t["plan_ms"] for t in batched[1]) / args.steps
+        # the graph's own time (HIP events around the launch); the rest of ms_per_step is host
+        # work: planning, descriptor copies, graph capture and instantiation, results
+        cfg["batched_device_ms_per_step"] = sum(t["total_ms"] for t in batched[1]) / args.steps
     out = {
         "metric": METRIC, "value": value, "unit": "primes tested/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
