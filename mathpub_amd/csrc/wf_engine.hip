@@ -99,7 +99,8 @@ __constant__ u32 c_WCLS[8];
 
 // The wheel tables, copied to LDS by every workgroup that looks them up per lane
 // (per-lane lookups in constant memory are vector memory loads with cache latency).
-struct WheelLds {
+// m30 leads, 8-byte aligned, so that the eight m's of one (qi, o mod 30) are one read.
+struct __attribute__((aligned(8))) WheelLds {
   unsigned char m30[8 * 30 * 8];
   unsigned char ws[8 * 30];
   u32 wcls[8];
@@ -121,15 +122,25 @@ __device__ __forceinline__ void load_wheel(WheelLds& w) {
   }
 }
 
+// R30 packed, byte c = R30[c].
+#define R30_PACKED 0x1D1713110D0B0701ull
+
+// The eight m's of c_M30[qi][o mod 30] (byte c for class c) in one 64-bit LDS read, so that a
+// prime's eight class starts wait on one lookup instead of eight.
+__device__ __forceinline__ u64 class_ms(u64 o, int qi, const WheelLds& w) {
+  return *reinterpret_cast<const u64*>(&w.m30[(qi * 30 + (u32)o % 30u) * 8]);
+}
+
 // Byte index j (relative to the segment base) of the first mark of class c for a prime q
-// whose first multiple at or after max(base, q^2) lies at offset o (numbers) from base.
-// o < 2^32 at every call (o < q, or o < the span of a segment or fill run).  The
-// numerator is a multiple of 30, so below 30 * 2^32 the division is exact: halve, then
-// multiply by 15^-1 mod 2^32.  Larger starts lie beyond every segment and run.
-__device__ __forceinline__ u64 class_start(u64 o, u32 q, int qi, int c, const WheelLds& w) {
+// whose first multiple at or after max(base, q^2) lies at offset o (numbers) from base;
+// ms = class_ms(o, qi).  o < 2^32 at every call (o < q, or o < the span of a segment or
+// fill run).  The numerator is a multiple of 30, so below 30 * 2^32 the division is exact:
+// halve, then multiply by 15^-1 mod 2^32.  Larger starts lie beyond every segment and run.
+__device__ __forceinline__ u64 class_start(u64 o, u32 q, u64 ms, int c) {
   const u32 o32 = (u32)o;
-  const u32 m = w.m30[(qi * 30 + o32 % 30u) * 8 + c];
-  const u64 v = (u64)m * q + o32 - w.r30[c];
+  const u32 m = (u32)(ms >> (8 * c)) & 0xFFu;
+  const u32 r = (u32)(R30_PACKED >> (8 * c)) & 0xFFu;
+  const u64 v = (u64)m * q + o32 - r;
   if (v >= (30ull << 32)) return ~0ull;   // beyond any segment or run (q close to 2^32)
   return (u64)((u32)(v >> 1) * 0xEEEEEEEFu);
 }
@@ -277,8 +288,9 @@ __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __r
         wheel_walk<u64>(o, q, qi, span, wl, [&](u32 j, int c) { hit(fr.lseg0 + (j >> sbshift), j, (u32)c); });
         continue;
       }
+      const u64 ms = class_ms(o, qi, wl);
       for (int c = c0; c < c1; c++) {
-        u64 j = class_start(o, q, qi, c, wl);
+        u64 j = class_start(o, q, ms, c);
         for (; j + 3ull * q < nbytes; j += 4ull * q) {
           u32 sg[4], jj[4];
 #pragma unroll
@@ -431,7 +443,7 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
       u64 o;
       if (!first_offset(d.base, q, minv, span, o)) continue;
       const int qi = wl.qi[q % 30];
-      const u32 jl = (u32)class_start(o, q, qi, lane & 7, wl);     // lane c (< 8) holds class c's start
+      const u32 jl = (u32)class_start(o, q, class_ms(o, qi, wl), lane & 7);   // lane c (< 8): class c's start
       for (int c = 0; c < 8; c++) {
         const u32 j0 = __shfl(jl, c, 64);
         for (u32 j = j0 + (u32)lane * q; j < d.nbytes; j += 64u * q)
@@ -460,9 +472,10 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
           else wheel_walk<u64>(o, q, qi, span, wl, mark);
           continue;
         }
+        const u64 ms = class_ms(o, qi, wl);
 #pragma unroll 1
         for (int c = 0; c < 8; c++) {
-          for (u32 j = (u32)class_start(o, q, qi, c, wl); j < d.nbytes; j += q)
+          for (u32 j = (u32)class_start(o, q, ms, c); j < d.nbytes; j += q)
             atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c));
         }
       }
@@ -1690,10 +1703,6 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
     // count is a sum of independent 0/1 (or floor/ceil) terms, so its deviation is ~sqrt(E)
     const double E = 8.0 * e->cfg.seg_bytes * (e->h_inv_sum[P.maxnq] - e->h_inv_sum[e->q1idx]);
     P.bcap = (u32)((E + 10.0 * std::sqrt(E) + 1024.0 + 63.0) / 64.0) * 64;
-    // fill work items.  Primes in [Q, 2Q) walk runs of G segments, G the power of two
-    // (<= RUN_MAX) nearest 64 Q / (8 seg_bytes): about 64 records per prime and run, so
-    // the per-run start of a prime is paid for by its hits.  Each run's primes are cut
-    // into slices of ~FILL_TARGET expected records.
     // fill groups.  Primes in [Q, 2Q) use groups of at most G segments, G the power of two
     // (<= RUN_MAX) nearest 64 Q / (8 seg_bytes): about 64 records per prime and group, so a
     // prime's start in a group is paid for by its hits.  A group is a run of one range cut
