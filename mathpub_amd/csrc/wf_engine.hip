@@ -956,13 +956,25 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
 #ifndef PSP_WAVES_EU
 #define PSP_WAVES_EU 6
 #endif
+#ifndef PSP_ORDER_MAX
+#define PSP_ORDER_MAX 16384  // the order sieve takes the primes 101 <= p < PSP_ORDER_MAX (< 2^15)
+#endif
+
+// One prime of the order sieve.  A base-2 strong pseudoprime n (indeed any Fermat one) has
+// 2^(n-1) = 1 (mod p) for each prime factor p, so ord_p(2) | n - 1.  With lcm(2, ord_p(2)) =
+// s2, an odd multiple n = m p can be one only if m = u (mod s2), u = p^-1 mod s2 (p m = 1
+// (mod s2)): every other odd multiple of p is ruled out.  mod = p s2 < 2^32 with its Barrett
+// constant; rp = 1/p rounded to float.
+struct PspOrder { u32 p, s2, u, mod; u64 minv; float rp; u32 pad; };
 
 __attribute__((amdgpu_waves_per_eu(PSP_WAVES_EU)))
 __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u32 nseg,
                                              const u32* __restrict__ bitmap, u32 bmw, u32 tps,
+                                             const PspOrder* __restrict__ po, u32 npo,
                                              u64* __restrict__ out, u64 cap, unsigned long long* __restrict__ count,
                                              DevAcc* __restrict__ acc) {
   __shared__ u64 queue[PSP_TILE];
+  __shared__ u32 ruled[PSP_TILE / 32];   // bit i: odd number i of the tile ruled out by the order sieve
   __shared__ u32 qn;
   __shared__ signed char w30[32];
   if (threadIdx.x < 30) w30[threadIdx.x] = c_W30[threadIdx.x];   // ordered by the first barrier below
@@ -975,13 +987,38 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
     const u32 nodd = 15u * d.nbytes;                 // odd offsets 1, 3, ..., 30 nbytes - 1
     if (k * PSP_TILE >= nodd) continue;              // uniform across the workgroup
     if (threadIdx.x == 0) qn = 0;
+    for (u32 w = threadIdx.x; w < PSP_TILE / 32; w += blockDim.x) ruled[w] = 0;
+    __syncthreads();
+    // order sieve over the tile's odd numbers A, A + 2, ..., A + 2 (PSP_TILE - 1): for each
+    // prime, the first odd multiple m p >= A from R = A mod (p s2), then every odd multiple
+    // with m != u (mod s2) is ruled out
+    {
+      const u64 A = d.base + 2ull * k * PSP_TILE + 1;
+      for (u32 f = threadIdx.x; f < npo; f += blockDim.x) {
+        const PspOrder F = po[f];
+        const u32 R = mod_barrett(A, F.mod, F.minv);
+        u32 q = (u32)((float)R * F.rp);              // floor(R / p) within a step or two
+        while (q * F.p > R) q--;
+        while ((q + 1) * F.p <= R) q++;
+        u32 m = q + (q * F.p != R);                  // ceil(R / p) <= s2
+        m |= 1u;                                     // odd: A - R is an even multiple of p
+        u32 off = m * F.p - R;                       // A + off = (A - R) + m p, off < 2p
+        u32 mm = m >= F.s2 ? m - F.s2 : m;           // m mod s2
+        for (; off < 2u * PSP_TILE; off += 2u * F.p) {
+          if (mm != F.u) atomicOr(&ruled[off >> 6], 1u << ((off >> 1) & 31u));
+          mm += 2u;
+          if (mm >= F.s2) mm -= F.s2;
+        }
+      }
+    }
     __syncthreads();
     for (u32 j0 = 0; j0 < PSP_TILE; j0 += blockDim.x) {
-      const u32 i = k * PSP_TILE + j0 + threadIdx.x;
+      const u32 li = j0 + threadIdx.x;
+      const u32 i = k * PSP_TILE + li;
       const u32 off = 2u * i + 1u;
       const u64 n = d.base + off;
       bool want = false;
-      if (i < nodd && off >= d.lo_off && off < d.hi_off) {
+      if (i < nodd && off >= d.lo_off && off < d.hi_off && !((ruled[li >> 5] >> (li & 31u)) & 1u)) {
         const int cls = w30[off % 30u];
         const bool comp = cls < 0 ? n > 5
                                   : ((bitmap[(u64)sg * bmw + off / 120u] >> (((off / 30u) & 3u) * 8u + (u32)cls)) & 1u);
@@ -1183,6 +1220,8 @@ struct wf_engine {
   unsigned long long* d_ctr = nullptr;   // [1] events, [2] strong pseudoprimes, [3] proved primes
   u64* d_psp = nullptr;                  // strong pseudoprimes found
   u64 psp_cap = 0;
+  PspOrder* d_porder = nullptr;          // the order sieve's primes (built on first use)
+  u32 nporder = 0;
   hipEvent_t ev[4] = {};
   wf_timing last;
   struct Server* srv = nullptr;
@@ -1455,6 +1494,7 @@ extern "C" void wf_close(wf_engine* e) {
     (void)hipFree(e->bs[k].tcnt);
   }
   (void)hipFree(e->d_psp);
+  (void)hipFree(e->d_porder);
   (void)hipFree(e->d_stats);
   (void)hipFree(e->d_pre);
   (void)hipFree(e->d_flags);
@@ -1850,6 +1890,52 @@ static double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+static u32 powmod32(u32 b, u32 x, u32 m) {
+  u64 r = 1, y = b % m;
+  for (; x; x >>= 1) {
+    if (x & 1) r = r * y % m;
+    y = y * y % m;
+  }
+  return (u32)r;
+}
+
+// The order sieve's table (k_psp): every prime 101 <= p < PSP_ORDER_MAX with its
+// multiplicative order of 2 (p - 1 stripped of each prime factor f while 2^(L/f) = 1 mod p),
+// s2 = lcm(2, ord), u = p^-1 mod s2 and the Barrett constant of p s2.  Built once.
+static int ensure_porder(wf_engine* e) {
+  if (e->d_porder) return WF_OK;
+  std::vector<PspOrder> t;
+  for (u32 p = 101; p < PSP_ORDER_MAX; p += 2) {
+    bool prime = true;
+    for (u32 d = 3; d * d <= p; d += 2)
+      if (p % d == 0) { prime = false; break; }
+    if (!prime) continue;
+    u32 L = p - 1, x = p - 1;
+    for (u32 f = 2; f <= x; f++) {
+      if (x % f) continue;
+      while (x % f == 0) x /= f;
+      while (L % f == 0 && powmod32(2, L / f, p) == 1) L /= f;
+    }
+    PspOrder F;
+    F.p = p;
+    F.s2 = L % 2 ? 2 * L : L;
+    F.u = 1;
+    while ((u64)F.u * p % F.s2 != 1) F.u++;
+    F.mod = p * F.s2;
+    F.minv = 0xFFFFFFFFFFFFFFFFull / F.mod;
+    F.rp = 1.0f / (float)p;
+    F.pad = 0;
+    t.push_back(F);
+  }
+  if (hipMalloc(&e->d_porder, t.size() * sizeof(PspOrder)) != hipSuccess) {
+    e->d_porder = nullptr;
+    return WF_E_NOMEM;
+  }
+  HIPCHK(hipMemcpy(e->d_porder, t.data(), t.size() * sizeof(PspOrder), hipMemcpyHostToDevice));
+  e->nporder = (u32)t.size();
+  return WF_OK;
+}
+
 static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith, const TestOut& to,
                       bool want_psp = false) {
   PassPlan P;
@@ -1861,6 +1947,7 @@ static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith,
   if ((rc = size_set(e->bs[0], P)) != WF_OK) return rc;
   if (want_psp && (rc = grow(e->bs[0].bitmap, e->bs[0].bitmap_cap, P.segs.size() * (e->cfg.seg_bytes / 4))) != WF_OK)
     return rc;
+  if (want_psp && (rc = ensure_porder(e)) != WF_OK) return rc;
   if ((rc = upload_pass(e, P)) != WF_OK) return rc;
   HIPCHK(hipEventRecord(e->ev[0], e->st));
   if ((rc = enqueue_sieve(e, P, e->d_segs, e->d_runs, e->d_fills, e->bs[0], e->st, to.acc, want_psp)) != WF_OK) return rc;
@@ -1871,8 +1958,8 @@ static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith,
     const u32 tps = (15u * e->cfg.seg_bytes + PSP_TILE - 1) / PSP_TILE;
     const u64 nt = P.segs.size() * (u64)tps;
     hipLaunchKernelGGL(k_psp, dim3((u32)std::min<u64>(nt, (u64)e->prop.multiProcessorCount * 16)), dim3(256), 0, e->st,
-                       e->d_segs, (u32)P.segs.size(), e->bs[0].bitmap, e->cfg.seg_bytes / 4, tps, e->d_psp,
-                       e->psp_cap, e->d_ctr + 2, to.acc);
+                       e->d_segs, (u32)P.segs.size(), e->bs[0].bitmap, e->cfg.seg_bytes / 4, tps, e->d_porder,
+                       e->nporder, e->d_psp, e->psp_cap, e->d_ctr + 2, to.acc);
     HIPCHK(hipGetLastError());
   }
   u32 flags = 0;

@@ -154,9 +154,10 @@ def test_autotune_batch_keeps_results():
 
 
 @pytest.mark.parametrize("seg,block,bmin", [(16384, 256, 512), (65536, 1024, 262144), (131072, 1024, 524288),
-                                            (131072, 512, 32768)])
+                                            (131072, 512, 32768), (65536, 1024, 1 << 25)])
 def test_segment_configs(seg, block, bmin):
-    """Every LDS segment size, sieve workgroup and bucket threshold gives the same results."""
+    """Every LDS segment size, sieve workgroup and bucket threshold gives the same results.
+    A threshold of 2^25 puts primes above 2^24 in the direct tier (the 64-bit wheel walk)."""
     e = wf.Engine(seg_bytes=seg, sieve_block=block, bucket_min=bmin)
     try:
         for lo, hi, psp in [(3, 2 * 10 ** 7, False), (10 ** 12 + 12345, 10 ** 12 + 2 * 10 ** 8 + 7, False),
