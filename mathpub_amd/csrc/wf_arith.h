@@ -647,10 +647,6 @@ __device__ __forceinline__ void m28_powc(u64 p, u64 e, int ebits, const u32* pw,
   m28_out(x, m.n, m.ninv, r);
 }
 
-}  // namespace wf
-
-namespace wf {
-
 // ---------------------------------------------------------------------------------
 // base-2 strong probable-prime test of an odd n >= 3, n < 2^64
 // ---------------------------------------------------------------------------------
@@ -919,6 +915,75 @@ __device__ __forceinline__ u32 bpsw_flags(u64 n) {
   if (n < 3 || !(n & 1)) return n == 2 ? 4u : 0u;
   const u32 f = (sprp2_64(n) ? 1u : 0u) | (slprp_64(n) ? 2u : 0u);
   return f == 3u ? 7u : f;
+}
+
+// ---------------------------------------------------------------------------------
+// order sieve for the SPRP-2 pass
+// ---------------------------------------------------------------------------------
+
+// x mod q for x < 2^64 by Barrett with minv = floor((2^64 - 1) / q)
+__device__ __forceinline__ u32 mod_barrett(u64 x, u32 q, u64 minv) {
+  const u64 t = __umul64hi(x, minv);
+  u64 r = x - t * q;
+  if (r >= q) r -= q;
+  return (u32)r;
+}
+
+// One prime of the order sieve.  A base-2 strong pseudoprime n (indeed any Fermat one) has
+// 2^(n-1) = 1 (mod p) for each prime factor p, so ord_p(2) | n - 1.  With lcm(2, ord_p(2)) =
+// s2, an odd multiple n = m p can be one only if m = u (mod s2), u = p^-1 mod s2 (p m = 1
+// (mod s2)): every other odd multiple of p is ruled out.  mod = p s2 < 2^32 (p < 2^15) with
+// its Barrett constant; rp = 1/p rounded to float.
+struct PspOrder { u32 p, s2, u, mod; u64 minv; float rp; u32 pad; };
+
+__device__ __forceinline__ u32 powmod32(u32 b, u32 x, u32 m) {
+  u64 r = 1, y = b % m;
+  for (; x; x >>= 1) {
+    if (x & 1) r = r * y % m;
+    y = y * y % m;
+  }
+  return (u32)r;
+}
+
+// The entry of an odd prime 3 < p < 2^15 (built on the host): ord_p(2) is p - 1 stripped of
+// each prime factor f while 2^(L/f) = 1 (mod p).
+__device__ __forceinline__ PspOrder psp_order_entry(u32 p) {
+  u32 L = p - 1, x = p - 1;
+  for (u32 f = 2; f <= x; f++) {
+    if (x % f) continue;
+    while (x % f == 0) x /= f;
+    while (L % f == 0 && powmod32(2, L / f, p) == 1) L /= f;
+  }
+  PspOrder F;
+  F.p = p;
+  F.s2 = L % 2 ? 2 * L : L;
+  F.u = 1;
+  while ((u64)F.u * p % F.s2 != 1) F.u++;
+  F.mod = p * F.s2;
+  F.minv = 0xFFFFFFFFFFFFFFFFull / F.mod;
+  F.rp = 1.0f / (float)p;
+  F.pad = 0;
+  return F;
+}
+
+// The odd numbers A, A + 2, ..., A + 2 (nodd - 1) (A odd, nodd <= 2^30) that the prime F
+// rules out: ruled(i) for each, i its index.  The first odd multiple m p >= A comes from
+// R = A mod (p s2): A - R is an even multiple of p, so m = ceil(R / p) made odd; then the
+// walk keeps m mod s2 in a counter.
+template <class G>
+__device__ __forceinline__ void psp_order_walk(u64 A, const PspOrder& F, u32 nodd, G ruled) {
+  const u32 R = mod_barrett(A, F.mod, F.minv);
+  u32 q = (u32)((float)R * F.rp);              // floor(R / p) within a step or two
+  while (q * F.p > R) q--;
+  while ((q + 1) * F.p <= R) q++;
+  u32 m = (q + (q * F.p != R)) | 1u;           // the first odd cofactor, <= s2 + 1
+  u32 off = m * F.p - R;                       // A + off = (A - R) + m p, off < 2p
+  u32 mm = m >= F.s2 ? m - F.s2 : m;           // m mod s2
+  for (; off < 2u * nodd; off += 2u * F.p) {
+    if (mm != F.u) ruled(off >> 1);
+    mm += 2u;
+    if (mm >= F.s2) mm -= F.s2;
+  }
 }
 
 }  // namespace wf

@@ -71,14 +71,6 @@ __device__ __forceinline__ u32 wave_incl_scan(u32 v) {
   return v;
 }
 
-// x mod q for x < 2^64 by Barrett with minv = floor((2^64 - 1) / q)
-__device__ __forceinline__ u32 mod_barrett(u64 x, u32 q, u64 minv) {
-  const u64 t = __umul64hi(x, minv);
-  u64 r = x - t * q;
-  if (r >= q) r -= q;
-  return (u32)r;
-}
-
 __constant__ u32 c_R30[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 // c_M30[qi][o mod 30][c]: the m in [0, 30) with o + m q = R30[c] (mod 30), qi = the class
 // of q mod 30 (an index into R30).
@@ -960,12 +952,6 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
 #define PSP_ORDER_MAX 16384  // the order sieve takes the primes 101 <= p < PSP_ORDER_MAX (< 2^15)
 #endif
 
-// One prime of the order sieve.  A base-2 strong pseudoprime n (indeed any Fermat one) has
-// 2^(n-1) = 1 (mod p) for each prime factor p, so ord_p(2) | n - 1.  With lcm(2, ord_p(2)) =
-// s2, an odd multiple n = m p can be one only if m = u (mod s2), u = p^-1 mod s2 (p m = 1
-// (mod s2)): every other odd multiple of p is ruled out.  mod = p s2 < 2^32 with its Barrett
-// constant; rp = 1/p rounded to float.
-struct PspOrder { u32 p, s2, u, mod; u64 minv; float rp; u32 pad; };
 
 __attribute__((amdgpu_waves_per_eu(PSP_WAVES_EU)))
 __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u32 nseg,
@@ -989,27 +975,11 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
     if (threadIdx.x == 0) qn = 0;
     for (u32 w = threadIdx.x; w < PSP_TILE / 32; w += blockDim.x) ruled[w] = 0;
     __syncthreads();
-    // order sieve over the tile's odd numbers A, A + 2, ..., A + 2 (PSP_TILE - 1): for each
-    // prime, the first odd multiple m p >= A from R = A mod (p s2), then every odd multiple
-    // with m != u (mod s2) is ruled out
+    // order sieve over the tile's odd numbers (psp_order_walk per prime)
     {
       const u64 A = d.base + 2ull * k * PSP_TILE + 1;
-      for (u32 f = threadIdx.x; f < npo; f += blockDim.x) {
-        const PspOrder F = po[f];
-        const u32 R = mod_barrett(A, F.mod, F.minv);
-        u32 q = (u32)((float)R * F.rp);              // floor(R / p) within a step or two
-        while (q * F.p > R) q--;
-        while ((q + 1) * F.p <= R) q++;
-        u32 m = q + (q * F.p != R);                  // ceil(R / p) <= s2
-        m |= 1u;                                     // odd: A - R is an even multiple of p
-        u32 off = m * F.p - R;                       // A + off = (A - R) + m p, off < 2p
-        u32 mm = m >= F.s2 ? m - F.s2 : m;           // m mod s2
-        for (; off < 2u * PSP_TILE; off += 2u * F.p) {
-          if (mm != F.u) atomicOr(&ruled[off >> 6], 1u << ((off >> 1) & 31u));
-          mm += 2u;
-          if (mm >= F.s2) mm -= F.s2;
-        }
-      }
+      for (u32 f = threadIdx.x; f < npo; f += blockDim.x)
+        psp_order_walk(A, po[f], PSP_TILE, [&](u32 i) { atomicOr(&ruled[i >> 5], 1u << (i & 31u)); });
     }
     __syncthreads();
     for (u32 j0 = 0; j0 < PSP_TILE; j0 += blockDim.x) {
@@ -1890,18 +1860,8 @@ static double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-static u32 powmod32(u32 b, u32 x, u32 m) {
-  u64 r = 1, y = b % m;
-  for (; x; x >>= 1) {
-    if (x & 1) r = r * y % m;
-    y = y * y % m;
-  }
-  return (u32)r;
-}
-
-// The order sieve's table (k_psp): every prime 101 <= p < PSP_ORDER_MAX with its
-// multiplicative order of 2 (p - 1 stripped of each prime factor f while 2^(L/f) = 1 mod p),
-// s2 = lcm(2, ord), u = p^-1 mod s2 and the Barrett constant of p s2.  Built once.
+// The order sieve's table (k_psp): every prime 101 <= p < PSP_ORDER_MAX (psp_order_entry).
+// Built once.
 static int ensure_porder(wf_engine* e) {
   if (e->d_porder) return WF_OK;
   std::vector<PspOrder> t;
@@ -1909,23 +1869,7 @@ static int ensure_porder(wf_engine* e) {
     bool prime = true;
     for (u32 d = 3; d * d <= p; d += 2)
       if (p % d == 0) { prime = false; break; }
-    if (!prime) continue;
-    u32 L = p - 1, x = p - 1;
-    for (u32 f = 2; f <= x; f++) {
-      if (x % f) continue;
-      while (x % f == 0) x /= f;
-      while (L % f == 0 && powmod32(2, L / f, p) == 1) L /= f;
-    }
-    PspOrder F;
-    F.p = p;
-    F.s2 = L % 2 ? 2 * L : L;
-    F.u = 1;
-    while ((u64)F.u * p % F.s2 != 1) F.u++;
-    F.mod = p * F.s2;
-    F.minv = 0xFFFFFFFFFFFFFFFFull / F.mod;
-    F.rp = 1.0f / (float)p;
-    F.pad = 0;
-    t.push_back(F);
+    if (prime) t.push_back(psp_order_entry(p));
   }
   if (hipMalloc(&e->d_porder, t.size() * sizeof(PspOrder)) != hipSuccess) {
     e->d_porder = nullptr;

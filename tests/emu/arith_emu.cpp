@@ -127,4 +127,19 @@ void emu_m28(uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
   *hi = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
 }
 
+// The order sieve of the SPRP-2 pass over the odd numbers A, A + 2, ..., A + 2 (nodd - 1):
+// out[i] = 1 where some prime 101 <= p < pmax rules A + 2 i out (psp_order_entry and
+// psp_order_walk, as k_psp runs them).
+void emu_psp_order(uint64_t A, uint32_t pmax, uint32_t nodd, uint8_t* out) {
+  for (uint32_t i = 0; i < nodd; i++) out[i] = 0;
+  for (uint32_t p = 101; p < pmax; p += 2) {
+    bool prime = true;
+    for (uint32_t d = 3; d * d <= p; d += 2)
+      if (p % d == 0) { prime = false; break; }
+    if (!prime) continue;
+    const PspOrder F = psp_order_entry(p);
+    psp_order_walk(A, F, nodd, [&](uint32_t i) { out[i] = 1; });
+  }
 }
+
+}  // extern "C"

@@ -29,6 +29,7 @@ def emu():
     L.emu_fp64x2.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P, ctypes.c_int]
     L.emu_fp64c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P, ctypes.c_int]
     L.emu_m28c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, P, P]
+    L.emu_psp_order.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8)]
     return L
 
 
@@ -379,3 +380,37 @@ def test_bpsw_flags(emu_lucas):
     for _ in range(3000):
         n = rng.getrandbits(64) | 1
         assert bool(emu_lucas.emu_bpsw(n) & 4) == sympy.isprime(n), n
+
+
+def test_psp_order_sieve(emu):
+    """k_psp's order sieve against its definition: an odd n is ruled out exactly when some
+    prime 101 <= p < 16384 divides it with ord_p(2) not dividing n - 1 (orders from sympy's
+    n_order, multiples found by plain division).  Tiles near 2^64, 2^50 and small ones where
+    n = p itself appears (never ruled out); every base-2 Fermat pseudoprime the tiles hold,
+    and the listed strong pseudoprimes, are kept."""
+    from sympy import n_order, primerange
+    PMAX, NODD = 16384, 2048
+    orders = [(p, n_order(2, p)) for p in primerange(101, PMAX)]
+    rng = random.Random(41)
+    tiles = [1, 101, 16383, 3 * 10 ** 9 + 1] + [(1 << 64) - 2 * NODD - 1, (1 << 64) - 4095] + \
+        [rng.randrange(1, 1 << 64) | 1 for _ in range(6)] + [rng.randrange(1 << 49, 1 << 50) | 1 for _ in range(4)]
+    tiles += [n - 2 * rng.randrange(NODD) for n in (2047, 3277, 4681, 15841, 29341, 52633, 3825123056546413051,
+                                                     18446744066047760377)]
+    out = (ctypes.c_uint8 * NODD)()
+    for A in tiles:
+        A = max(A, 1) | 1
+        A = min(A, (1 << 64) - 2 * NODD + 1)
+        emu.emu_psp_order(A, PMAX, NODD, out)
+        want = [0] * NODD
+        for p, L in orders:
+            n = -(-A // p) * p
+            while n < A + 2 * NODD:
+                if n % 2 and (n - 1) % L:
+                    want[(n - A) // 2] = 1
+                n += p
+        assert list(out) == want, A
+        for i in range(NODD):
+            n = A + 2 * i
+            if out[i] and n > 1:
+                assert pow(2, n - 1, n) != 1, n
+
