@@ -24,6 +24,7 @@
 // give the same results as the GPU's v_fma_f64 / v_rndne_f64 / v_ldexp_f64.
 #include <cmath>
 #define __device__
+#define __host__
 #define __forceinline__ inline
 static inline uint64_t __umul64hi(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) >> 64); }
 #define __builtin_fma(a, b, c) std::fma(a, b, c)
@@ -936,7 +937,7 @@ __device__ __forceinline__ u32 mod_barrett(u64 x, u32 q, u64 minv) {
 // its Barrett constant; rp = 1/p rounded to float.
 struct PspOrder { u32 p, s2, u, mod; u64 minv; float rp; u32 pad; };
 
-__device__ __forceinline__ u32 powmod32(u32 b, u32 x, u32 m) {
+__host__ __device__ __forceinline__ u32 powmod32(u32 b, u32 x, u32 m) {
   u64 r = 1, y = b % m;
   for (; x; x >>= 1) {
     if (x & 1) r = r * y % m;
@@ -947,7 +948,7 @@ __device__ __forceinline__ u32 powmod32(u32 b, u32 x, u32 m) {
 
 // The entry of an odd prime 3 < p < 2^15 (built on the host): ord_p(2) is p - 1 stripped of
 // each prime factor f while 2^(L/f) = 1 (mod p).
-__device__ __forceinline__ PspOrder psp_order_entry(u32 p) {
+__host__ __device__ __forceinline__ PspOrder psp_order_entry(u32 p) {
   u32 L = p - 1, x = p - 1;
   for (u32 f = 2; f <= x; f++) {
     if (x % f) continue;
