@@ -84,8 +84,9 @@ typedef struct { uint64_t lo, hi; } wf_range;   /* [lo, hi), hi == 0 means 2^64 
 /* Tuning knobs; a zero field means "engine default". */
 typedef struct {
     uint32_t seg_bytes;     /* LDS sieve segment, 4096 .. 131072 (a power of two) */
-    uint32_t sieve_block;   /* threads per sieve workgroup (256 .. 1024) */
-    uint32_t test_block;    /* threads per test workgroup (64 .. 1024) */
+    uint32_t sieve_block;   /* threads per sieve workgroup (64 .. 1024, a multiple of 64) */
+    uint32_t test_block;    /* threads per test workgroup: 256, the only size the test
+                               kernels are built for (others give WF_E_ARG) */
     uint32_t arith;         /* 0 = fastest exact path, 1 = force 32-bit Montgomery */
     uint32_t bucket_min;    /* sieving primes >= this go through bucket records (>= 512) */
     uint64_t chunk;         /* numbers per device pass of wf_scan (memory bound) */

@@ -120,7 +120,9 @@ __device__ __forceinline__ void load_wheel(WheelLds& w) {
 // The eight m's of c_M30[qi][o mod 30] (byte c for class c) in one 64-bit LDS read, so that a
 // prime's eight class starts wait on one lookup instead of eight.
 __device__ __forceinline__ u64 class_ms(u64 o, int qi, const WheelLds& w) {
-  return *reinterpret_cast<const u64*>(&w.m30[(qi * 30 + (u32)o % 30u) * 8]);
+  u64 ms;
+  __builtin_memcpy(&ms, &w.m30[(qi * 30 + (u32)o % 30u) * 8], sizeof ms);   // 8-byte aligned
+  return ms;
 }
 
 // Byte index j (relative to the segment base) of the first mark of class c for a prime q
