@@ -94,12 +94,20 @@ typedef struct {
                                fewer leave room for the next pass's sieve beside the tests */
     uint32_t prove;         /* partial sieve + proof for short pieces: WF_PROVE_AUTO (by a cost
                                model), WF_PROVE_OFF, WF_PROVE_ON (every piece above bucket_min^2) */
+    uint32_t derive;        /* composite bases from their prime factors' quotients (p < 2^50):
+                               WF_DERIVE_AUTO (whenever that needs fewer ladders than bases),
+                               WF_DERIVE_OFF (one ladder per base) */
 } wf_config;
 
 /* Prove mode: a piece is sieved by the primes below bucket_min only, and its survivors are
  * decided by BPSW (base-2 strong test + strong Lucas test, exact below 2^64) instead of
  * placing every sieving prime up to sqrt(hi) in it.  Results are identical either way. */
 enum { WF_PROVE_AUTO = 0, WF_PROVE_OFF = 1, WF_PROVE_ON = 2 };
+
+/* Composite bases: the Fermat quotient is additive, q_p(ab) = q_p(a) + q_p(b) (mod p) for
+ * p not dividing ab, so the quotients of every base follow from ladders for the distinct
+ * primes dividing the bases (the 17 "oeis" bases need 9).  Results are identical either way. */
+enum { WF_DERIVE_AUTO = 0, WF_DERIVE_OFF = 1 };
 
 /* Engine life cycle (the reference's gpu_init, wieferich_cuda.cu:1570-1577). */
 int  wf_open(int device, const wf_config *cfg, wf_engine **out);

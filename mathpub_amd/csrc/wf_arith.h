@@ -254,12 +254,13 @@ __device__ __forceinline__ void f64_mul_small_big(double& a, double& b, double c
   a = a1;
 }
 
-// Two ladders for a base b with window W: pw[d] = b^d < 2^32 for d < 2^W.  x = b^(leading
-// digit), then per window W squarings and one multiplication by b^d (BIG: both p >= 2^32).
-// The digits are left raw, as fp64_pow2x2 leaves them.
+// Two ladders with window W, for the base of pw0 on (p0, e0) and the base of pw1 on (p1, e1):
+// pw[d] = b^d < 2^32 for d < 2^W (one table twice for one base on two primes, or two tables
+// on one prime).  x = b^(leading digit), then per window W squarings and one multiplication
+// by b^d (BIG: both p >= 2^32).  The digits are left raw, as fp64_pow2x2 leaves them.
 template <int W, bool BIG = false>
-__device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64 e1, int ebits, const u32* pw,
-                                             double& a0, double& b0, double& a1, double& b1) {
+__device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64 e1, int ebits, const u32* pw0,
+                                             const u32* pw1, double& a0, double& b0, double& a1, double& b1) {
   F64Mod m0, m1;
   m0.p = pd0;
   m0.ip = 1.0 / pd0;
@@ -267,7 +268,7 @@ __device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64
   m1.ip = 1.0 / pd1;
   const int nw = (ebits + W - 1) / W;
   int sh = (nw - 1) * W;
-  const u32 c0 = pw[(int)(e0 >> sh)], c1 = pw[(int)(e1 >> sh)];
+  const u32 c0 = pw0[(int)(e0 >> sh)], c1 = pw1[(int)(e1 >> sh)];
   a0 = 1.0; b0 = 0.0; a1 = 1.0; b1 = 0.0;
   if ((double)c0 < pd0) a0 = (double)c0;
   else f64_mul_small(a0, b0, (double)c0, m0);
@@ -280,11 +281,11 @@ __device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64
       f64_sqr(a1, b1, m1);
     }
     if (BIG) {
-      f64_mul_small_big(a0, b0, (double)pw[(int)((e0 >> sh) & ((1u << W) - 1))], m0);
-      f64_mul_small_big(a1, b1, (double)pw[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
+      f64_mul_small_big(a0, b0, (double)pw0[(int)((e0 >> sh) & ((1u << W) - 1))], m0);
+      f64_mul_small_big(a1, b1, (double)pw1[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
     } else {
-      f64_mul_small(a0, b0, (double)pw[(int)((e0 >> sh) & ((1u << W) - 1))], m0);
-      f64_mul_small(a1, b1, (double)pw[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
+      f64_mul_small(a0, b0, (double)pw0[(int)((e0 >> sh) & ((1u << W) - 1))], m0);
+      f64_mul_small(a1, b1, (double)pw1[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
     }
   }
 }

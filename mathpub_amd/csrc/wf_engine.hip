@@ -242,6 +242,12 @@ struct DevEvent { u32 type, rid, j, pad; u64 p, q, r_hi, r_lo; };
 __constant__ u32 c_base[WF_MAXB + 1];
 __constant__ u32 c_bwin[WF_MAXB + 1];
 __constant__ u32 c_bpow[(WF_MAXB + 1) * BPOW];
+// Composite-base mode: the ladder primes l_s (ascending) with their powers l_s^d for the
+// common window, and the exponent of l_s in base j (c_lexp[j * WF_MAXL + s]).
+#define WF_MAXL 10
+__constant__ u32 c_lbase[WF_MAXL];
+__constant__ u32 c_lpow[WF_MAXL * BPOW];
+__constant__ unsigned char c_lexp[WF_MAXB * WF_MAXL];
 
 // first multiple offset of q at or after max(base, q^2), relative to base; returns false
 // if it lies at or beyond 'span' numbers
@@ -782,11 +788,11 @@ __device__ __forceinline__ void residue(u64 p, int eb, u32 b, int w, const u32* 
       fp64_pow2x2<5>(pd, e, 3.0, 2, eb, a0, b0, a1, b1);
     } else {
       switch (w) {
-        case 5: fp64_powc_x2<5>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
-        case 4: fp64_powc_x2<4>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
-        case 3: fp64_powc_x2<3>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
-        case 2: fp64_powc_x2<2>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
-        default: fp64_powc_x2<1>(pd, e, 3.0, 2, eb, pw, a0, b0, a1, b1); break;
+        case 5: fp64_powc_x2<5>(pd, e, 3.0, 2, eb, pw, pw, a0, b0, a1, b1); break;
+        case 4: fp64_powc_x2<4>(pd, e, 3.0, 2, eb, pw, pw, a0, b0, a1, b1); break;
+        case 3: fp64_powc_x2<3>(pd, e, 3.0, 2, eb, pw, pw, a0, b0, a1, b1); break;
+        case 2: fp64_powc_x2<2>(pd, e, 3.0, 2, eb, pw, pw, a0, b0, a1, b1); break;
+        default: fp64_powc_x2<1>(pd, e, 3.0, 2, eb, pw, pw, a0, b0, a1, b1); break;
       }
     }
     F64Mod m;
@@ -885,19 +891,19 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
           fp64_pow2x2<5>(pd0, p0 - 1, pd1, p1 - 1, eb, a0, b0, a1, b1);
         } else if (big) {
           switch (w) {
-            case 5: fp64_powc_x2<5, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 4: fp64_powc_x2<4, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 3: fp64_powc_x2<3, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 2: fp64_powc_x2<2, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            default: fp64_powc_x2<1, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 5: fp64_powc_x2<5, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            case 4: fp64_powc_x2<4, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            case 3: fp64_powc_x2<3, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            case 2: fp64_powc_x2<2, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            default: fp64_powc_x2<1, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
           }
         } else {
           switch (w) {
-            case 5: fp64_powc_x2<5>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 4: fp64_powc_x2<4>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 3: fp64_powc_x2<3>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            case 2: fp64_powc_x2<2>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
-            default: fp64_powc_x2<1>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, a0, b0, a1, b1); break;
+            case 5: fp64_powc_x2<5>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            case 4: fp64_powc_x2<4>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            case 3: fp64_powc_x2<3>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            case 2: fp64_powc_x2<2>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            default: fp64_powc_x2<1>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
           }
         }
         if (act0 && !divides_base(p0, b)) account_padic(o, d.rid, j, p0, pd0, a0, b0, s);
@@ -915,6 +921,130 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
     }
   }
   if (cur != 0xFFFFFFFFu) flush_sums(o, cur, curj, s);
+}
+
+// Composite-base mode (p < 2^50): the quotients of the ladder primes l_s first, two ladders
+// per lane on the lane's prime (fp64_powc_x2 with two power tables, window W for all), kept
+// in LDS; then for each base j, q_j = sum of e_js q_s (mod p), since q_p(ab) = q_p(a) +
+// q_p(b) (mod p) for p not dividing ab.  A pair whose ladder residue is not 1 mod p (not
+// for a prime p) is computed directly as k_test does.  Sums per base are reduced per
+// wavefront into LDS and flushed per range.
+template <int W>
+__attribute__((amdgpu_waves_per_eu(F64_WAVES_EU)))
+__global__ void __launch_bounds__(256) k_test_derived(const SegDesc* __restrict__ segs, u32 nseg,
+                                                      const u32* __restrict__ off, const u32* __restrict__ cnt,
+                                                      u32 pcap, u32 tps, TestOut o, u32 nl) {
+  __shared__ u32 lpw[WF_MAXL * BPOW];
+  __shared__ u64 qs[WF_MAXL][256];                 // the lane's prime: q of ladder s, ~0 if r != 1
+  __shared__ unsigned long long acks[WF_MAXB], asumq[WF_MAXB], atot[3];   // tests, near, flt
+  for (u32 k = threadIdx.x; k < nl * BPOW; k += blockDim.x) lpw[k] = c_lpow[k];
+  for (u32 k = threadIdx.x; k < o.nb; k += blockDim.x) acks[k] = asumq[k] = 0;
+  if (threadIdx.x < 3) atot[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  u32 cur = 0xFFFFFFFFu;
+  auto flush = [&]() {   // every thread, after a barrier
+    for (u32 k = threadIdx.x; k < o.nb; k += blockDim.x) {
+      if (acks[k] | asumq[k]) {
+        atomicAdd(&o.cks[(u64)cur * o.nb + k], acks[k]);
+        atomicAdd(&o.sumq[(u64)cur * o.nb + k], asumq[k]);
+      }
+      acks[k] = asumq[k] = 0;
+    }
+    if (threadIdx.x == 0) {
+      DevAcc* a = &o.acc[cur];
+      if (atot[0]) atomicAdd(&a->tests, atot[0]);
+      if (atot[1]) atomicAdd(&a->near, atot[1]);
+      if (atot[2]) atomicAdd(&a->flt, atot[2]);
+      atot[0] = atot[1] = atot[2] = 0;
+    }
+  };
+  const u64 nitems = (u64)nseg * tps;
+  for (u64 t = blockIdx.x; t < nitems; t += gridDim.x) {
+    const u32 sg = (u32)(t / tps);
+    const u32 beg = (u32)(t - (u64)sg * tps) * TEST_TILE;
+    const u32 n = cnt[sg];
+    if (beg >= n) continue;                       // uniform across the workgroup
+    const u32 end = min(n, beg + TEST_TILE);
+    const SegDesc d = segs[sg];
+    if (d.rid != cur) {
+      __syncthreads();
+      if (cur != 0xFFFFFFFFu) flush();
+      __syncthreads();
+      cur = d.rid;
+    }
+    const u32* po = off + (u64)sg * pcap;
+    const u64 etop = d.base + d.hi_off - 2;
+    const int eb = etop ? 64 - __builtin_clzll(etop) : 1;
+    const bool big = d.base >= (1ull << 32);
+    const u64 idle = big ? 4294967311ull : 3ull;
+    const double based = (double)d.base;          // p = base + offset < 2^50: exact as a double
+    for (u32 base = beg; base < end; base += blockDim.x) {
+      const u32 i = base + threadIdx.x;
+      const bool act = i < end;
+      const u32 f = act ? po[i] : 0;
+      const u64 p = act ? d.base + f : idle;
+      const double pd = act ? based + (double)f : (double)idle;
+      for (u32 sl = 0; sl < nl; sl += 2) {
+        const u32 s1 = sl + 1 < nl ? sl + 1 : sl;
+        double a0, b0, a1, b1;
+        if (big) fp64_powc_x2<W, true>(pd, p - 1, pd, p - 1, eb, lpw + sl * BPOW, lpw + s1 * BPOW, a0, b0, a1, b1);
+        else fp64_powc_x2<W>(pd, p - 1, pd, p - 1, eb, lpw + sl * BPOW, lpw + s1 * BPOW, a0, b0, a1, b1);
+        u64 q;
+        qs[sl][threadIdx.x] = f64_quotient(a0, b0, pd, q) ? q : ~0ull;
+        if (s1 != sl) qs[s1][threadIdx.x] = f64_quotient(a1, b1, pd, q) ? q : ~0ull;
+      }
+      for (u32 j = 0; j < o.nb; j++) {
+        const u32 b = c_base[j];
+        LaneSums ls = {0, 0, 0, 0, 0};
+        if (act && !divides_base(p, b)) {
+          u64 q = 0;
+          bool ok = true;
+          for (u32 sl = 0; sl < nl && ok; sl++) {
+            const u32 ex = c_lexp[j * WF_MAXL + sl];
+            if (!ex) continue;
+            const u64 v = qs[sl][threadIdx.x];
+            ok = v != ~0ull;
+            for (u32 k = 0; ok && k < ex; k++) {
+              q += v;
+              if (q >= p) q -= p;
+            }
+          }
+          if (ok) {
+            ls.tests = 1;
+            ls.cks = d_mix64(q ^ (p * GOLD));
+            ls.sumq = q;
+            if (q == 0) {
+              push_event(o, WF_EV_FOUND, cur, j, p, 0, 0, 1);
+            } else if (o.near_T > 0 && near_hit(p, q, o.near_T)) {
+              ls.near = 1;
+              const u64 lo = q * p, hi = __umul64hi(q, p);
+              const u64 rlo = lo + 1;
+              push_event(o, WF_EV_NEAR, cur, j, p, q, hi + (rlo < lo), rlo);
+            }
+          } else {
+            u64 rhi, rlo;
+            residue<A_F64>(p, eb, b, (int)c_bwin[j], c_bpow + j * BPOW, rhi, rlo);
+            account(o, cur, j, p, rhi, rlo, ls);
+          }
+        }
+        const u64 c = wave_sum(ls.cks), sq = wave_sum(ls.sumq);
+        const u64 tn = __popcll(__ballot(ls.tests != 0)), nn = __popcll(__ballot(ls.near != 0)),
+                  fn = __popcll(__ballot(ls.flt != 0));
+        if (lane == 0) {
+          if (c | sq) {
+            atomicAdd(&acks[j], (unsigned long long)c);
+            atomicAdd(&asumq[j], (unsigned long long)sq);
+          }
+          if (tn) atomicAdd(&atot[0], (unsigned long long)tn);
+          if (nn) atomicAdd(&atot[1], (unsigned long long)nn);
+          if (fn) atomicAdd(&atot[2], (unsigned long long)fn);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (cur != 0xFFFFFFFFu) flush();
 }
 
 // Quotients of arbitrary odd p (check / selftest) for the base in slot `slot` of the base
@@ -1156,6 +1286,8 @@ struct wf_engine {
   int sbshift = 15;
   int nb = 1;
   u32 bases[WF_MAXB] = {2};
+  int nl = 0;                    // composite-base mode: ladder primes (0 = one ladder per base)
+  int lw = 0;                    // and their common window
   unsigned long long* d_cks = nullptr;    // per range and base (checksums, sums of q)
   unsigned long long* d_sumq = nullptr;
   u64 cks_cap = 0, sumq_cap = 0;
@@ -1347,7 +1479,8 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
   if (e->cfg.test_wgs > 16 || e->cfg.seg_bytes < 4096 || (e->cfg.seg_bytes & (e->cfg.seg_bytes - 1)) ||
       e->cfg.seg_bytes > SEG_MAX || e->cfg.seg_bytes + 4096 > e->prop.sharedMemPerBlock || e->cfg.sieve_block < 64 ||
       e->cfg.sieve_block > 1024 || (e->cfg.sieve_block & 63) || e->cfg.test_block != 256 || e->cfg.arith > 1 ||
-      e->cfg.bucket_min < 512 || e->cfg.chunk < 30ull * e->cfg.seg_bytes || e->cfg.prove > WF_PROVE_ON) {
+      e->cfg.bucket_min < 512 || e->cfg.chunk < 30ull * e->cfg.seg_bytes || e->cfg.prove > WF_PROVE_ON ||
+      e->cfg.derive > WF_DERIVE_OFF) {
     delete e;
     return WF_E_ARG;
   }
@@ -1525,6 +1658,50 @@ static int put_base(u32 k, u32 b) {
   return WF_OK;
 }
 
+// Composite-base mode for a base list: the distinct primes dividing the bases become the
+// ladders when they are fewer than the bases (at most WF_MAXL), with one window for all
+// (the smallest of theirs) and each base's exponent of each.  nl = 0 otherwise.
+static int plan_ladders(wf_engine* e) {
+  e->nl = 0;
+  if (e->cfg.derive == WF_DERIVE_OFF || e->cfg.arith != 0) return WF_OK;
+  std::vector<u32> ls;
+  std::vector<std::vector<std::pair<u32, u32>>> fac(e->nb);   // (prime, exponent) per base
+  for (int j = 0; j < e->nb; j++) {
+    u32 x = e->bases[j];
+    for (u32 f = 2; (u64)f * f <= x; f++) {
+      u32 k = 0;
+      while (x % f == 0) { x /= f; k++; }
+      if (k) fac[j].push_back({f, k});
+    }
+    if (x > 1) fac[j].push_back({x, 1});
+    for (auto& fk : fac[j]) ls.push_back(fk.first);
+  }
+  std::sort(ls.begin(), ls.end());
+  ls.erase(std::unique(ls.begin(), ls.end()), ls.end());
+  if ((int)ls.size() >= e->nb || ls.size() > WF_MAXL) return WF_OK;
+  int w = 5;
+  u32 pw[BPOW];
+  for (u32 l : ls) w = std::min(w, base_window(l, pw));
+  std::vector<u32> lpow(WF_MAXL * BPOW, 0);
+  for (size_t s = 0; s < ls.size(); s++) {
+    base_window(ls[s], pw);                  // l^d for d below its own window, at least ours
+    for (int d = 0; d < (1 << w); d++) lpow[s * BPOW + d] = pw[d];
+  }
+  std::vector<unsigned char> lexp(WF_MAXB * WF_MAXL, 0);
+  for (int j = 0; j < e->nb; j++)
+    for (auto& fk : fac[j])
+      lexp[j * WF_MAXL + (std::lower_bound(ls.begin(), ls.end(), fk.first) - ls.begin())] = (unsigned char)fk.second;
+  std::vector<u32> lb(WF_MAXL, 0);
+  std::copy(ls.begin(), ls.end(), lb.begin());
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_lbase), lb.data(), WF_MAXL * sizeof(u32)) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_lpow), lpow.data(), lpow.size() * sizeof(u32)) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_lexp), lexp.data(), lexp.size()) != hipSuccess)
+    return WF_E_DEVICE;
+  e->nl = (int)ls.size();
+  e->lw = w;
+  return WF_OK;
+}
+
 extern "C" int wf_set_bases(wf_engine* e, const u32* bases, int nb) {
   if (!e || !bases || nb < 1 || nb > WF_MAXB) return WF_E_ARG;
   for (int i = 0; i < nb; i++)
@@ -1537,7 +1714,7 @@ extern "C" int wf_set_bases(wf_engine* e, const u32* bases, int nb) {
   }
   e->nb = nb;
   memcpy(e->bases, bases, (size_t)nb * sizeof(u32));
-  return WF_OK;
+  return plan_ladders(e);
 }
 
 // A sub-range of one arithmetic class: [lo, hi) (hi == 0 means 2^64) with its kernel.
@@ -1827,6 +2004,20 @@ static int enqueue_test(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, B
                         const TestOut& to, u32 wgs = 16) {
   const u64 nseg = P.segs.size();
   const u32 tgrid = (u32)std::min<u64>(nseg * P.tps, (u64)e->prop.multiProcessorCount * wgs);
+  if (arith == A_F64 && e->nl > 0) {   // composite bases from the ladder primes' quotients
+#define WF_TESTD(W) hipLaunchKernelGGL(k_test_derived<W>, dim3(tgrid), dim3(256), 0, st, dsegs, (u32)nseg, B.off, \
+                                       B.cnt, P.pcap, P.tps, to, (u32)e->nl)
+    switch (e->lw) {
+      case 5: WF_TESTD(5); break;
+      case 4: WF_TESTD(4); break;
+      case 3: WF_TESTD(3); break;
+      case 2: WF_TESTD(2); break;
+      default: WF_TESTD(1); break;
+    }
+#undef WF_TESTD
+    HIPCHK(hipGetLastError());
+    return WF_OK;
+  }
 #define WF_TEST(A) hipLaunchKernelGGL(k_test<A>, dim3(tgrid), dim3(256), 0, st, dsegs, (u32)nseg, B.off, B.cnt, \
                                       P.pcap, P.tps, to)
   switch (arith) {

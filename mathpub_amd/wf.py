@@ -12,6 +12,7 @@ LIB = os.path.join(PKG, "libwf.so")
 MAXB = 64
 EV_FOUND, EV_NEAR, EV_FLT = 0, 1, 2
 PROVE_AUTO, PROVE_OFF, PROVE_ON = 0, 1, 2
+DERIVE_AUTO, DERIVE_OFF = 0, 1     # composite bases from their prime factors' quotients
 BPSW_SPRP2, BPSW_LUCAS, BPSW_PRIME = 1, 2, 4
 
 u64, u32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64
@@ -40,7 +41,7 @@ class Range(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("seg_bytes", u32), ("sieve_block", u32), ("test_block", u32), ("arith", u32), ("bucket_min", u32),
-                ("chunk", u64), ("test_wgs", u32), ("prove", u32)]
+                ("chunk", u64), ("test_wgs", u32), ("prove", u32), ("derive", u32)]
 
 
 class Timing(ctypes.Structure):
@@ -123,9 +124,9 @@ class Engine:
     """One GPU engine (wf_open); not thread-safe, like the C object."""
 
     def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0, bucket_min=0, test_wgs=0,
-                 prove=PROVE_AUTO):
+                 prove=PROVE_AUTO, derive=DERIVE_AUTO):
         L = load()
-        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs, prove)
+        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs, prove, derive)
         h = ctypes.c_void_p()
         _check(L.wf_open(device, ctypes.byref(cfg), ctypes.byref(h)), "wf_open")
         self._h = h

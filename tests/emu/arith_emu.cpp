@@ -1,6 +1,7 @@
 // Host build of mathpub_amd/csrc/wf_arith.h for CPU tests of the kernel arithmetic.
 #define WF_HOST_EMU 1
 #include "wf_arith.h"
+#include <algorithm>
 
 using namespace wf;
 
@@ -79,11 +80,11 @@ void emu_fp64c(uint32_t base, uint64_t p0, uint64_t p1, int ebits, uint64_t* r, 
   double a0, b0, a1, b1;
 #define WF_POWC(B)                                                                              \
   switch (w) {                                                                                  \
-    case 5: fp64_powc_x2<5, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
-    case 4: fp64_powc_x2<4, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
-    case 3: fp64_powc_x2<3, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
-    case 2: fp64_powc_x2<2, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;        \
-    default: fp64_powc_x2<1, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, a0, b0, a1, b1); break;       \
+    case 5: fp64_powc_x2<5, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, pw, a0, b0, a1, b1); break;        \
+    case 4: fp64_powc_x2<4, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, pw, a0, b0, a1, b1); break;        \
+    case 3: fp64_powc_x2<3, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, pw, a0, b0, a1, b1); break;        \
+    case 2: fp64_powc_x2<2, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, pw, a0, b0, a1, b1); break;        \
+    default: fp64_powc_x2<1, B>(d0, p0 - 1, d1, p1 - 1, ebits, pw, pw, a0, b0, a1, b1); break;       \
   }
   if (big) {
     WF_POWC(true)
@@ -140,6 +141,31 @@ void emu_psp_order(uint64_t A, uint32_t pmax, uint32_t nodd, uint8_t* out) {
     const PspOrder F = psp_order_entry(p);
     psp_order_walk(A, F, nodd, [&](uint32_t i) { out[i] = 1; });
   }
+}
+
+// Two bases on one prime (the composite-base mode's ladder pairs): window W = the smaller of
+// the two bases' windows, r[0..3] for base A and r[4..7] for base B as finish() gives.
+void emu_fp64c2(uint32_t baseA, uint32_t baseB, uint64_t p, int ebits, uint64_t* r, int big) {
+  uint32_t pa[32], pb[32];
+  const int w = std::min(base_table(baseA, pa), base_table(baseB, pb));
+  const double d = (double)p;
+  double a0, b0, a1, b1;
+#define WF_POWC2(B)                                                                              \
+  switch (w) {                                                                                   \
+    case 5: fp64_powc_x2<5, B>(d, p - 1, d, p - 1, ebits, pa, pb, a0, b0, a1, b1); break;          \
+    case 4: fp64_powc_x2<4, B>(d, p - 1, d, p - 1, ebits, pa, pb, a0, b0, a1, b1); break;          \
+    case 3: fp64_powc_x2<3, B>(d, p - 1, d, p - 1, ebits, pa, pb, a0, b0, a1, b1); break;          \
+    case 2: fp64_powc_x2<2, B>(d, p - 1, d, p - 1, ebits, pa, pb, a0, b0, a1, b1); break;          \
+    default: fp64_powc_x2<1, B>(d, p - 1, d, p - 1, ebits, pa, pb, a0, b0, a1, b1); break;         \
+  }
+  if (big) {
+    WF_POWC2(true)
+  } else {
+    WF_POWC2(false)
+  }
+#undef WF_POWC2
+  finish(p, a0, b0, r);
+  finish(p, a1, b1, r + 4);
 }
 
 }  // extern "C"

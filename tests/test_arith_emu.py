@@ -30,6 +30,7 @@ def emu():
     L.emu_fp64c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P, ctypes.c_int]
     L.emu_m28c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, P, P]
     L.emu_psp_order.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8)]
+    L.emu_fp64c2.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, P, ctypes.c_int]
     return L
 
 
@@ -413,4 +414,27 @@ def test_psp_order_sieve(emu):
             n = A + 2 * i
             if out[i] and n > 1:
                 assert pow(2, n - 1, n) != 1, n
+
+
+def test_fp64_two_bases_one_prime(emu):
+    """The composite-base mode's ladder pairs: two bases on one prime with the smaller window
+    of the two (so 2 and 3 run at 4 bits, 29 at 2), p from 3 to 2^50, and the shorter window
+    step above 2^32."""
+    rng = random.Random(43)
+    pairs = [(2, 3), (5, 7), (11, 13), (17, 19), (23, 23), (2, 29), (3, 65521), ((1 << 31) - 1, 2), (6, 10)]
+    ps = [3, 5, 7, 11, 13, 29, 31, 1093, 3511, (1 << 50) - 27] + samples(2, 50, 800, 44)
+    out = (ctypes.c_uint64 * 8)()
+    for A, B in pairs:
+        for p in ps:
+            if p < 3:
+                continue
+            eb = min(50, (p - 1).bit_length() + rng.randint(0, 3))
+            for big in ([0, 1] if p >= 1 << 32 else [0]):
+                emu.emu_fp64c2(A, B, p, eb, out, big)
+                for k, b in ((0, A), (1, B)):
+                    r = out[4 * k] << 64 | out[4 * k + 1]
+                    assert r == pow(b, p - 1, p * p), (A, B, b, p, eb, big)
+                    assert out[4 * k + 2] == (r % p == 1), (b, p)
+                    if out[4 * k + 2]:
+                        assert out[4 * k + 3] == (r - 1) // p, (b, p)
 

@@ -311,6 +311,38 @@ def test_batch_any_bases_equals_single(feng):
                                                                          s["sumq_all"]), (lo, hi)
 
 
+@pytest.mark.parametrize("bases", [OEIS, [2, 3, 4, 6, 8, 9, 12, 16, 18, 24, 27, 32],
+                                   [6, 10, 14, 15, 21, 35, 1625, 65535, 4294967295],
+                                   [3, 5, 15, 30, 1626, 65537 * 3, 4294967291]])
+def test_composite_bases_derived_equal_direct(bases):
+    """Composite bases from their prime factors' quotients (the default whenever it needs
+    fewer ladders) against one ladder per base: bit-exact totals, sums and events, from p = 3
+    (p dividing ladder primes and bases) across 2^32 to the FP64 class end at 2^50, single
+    and batched."""
+    auto, off = wf.Engine(), wf.Engine(derive=wf.DERIVE_OFF)
+    try:
+        for e in (auto, off):
+            e.set_bases(bases)
+        for lo, hi in [(3, 3 * 10 ** 6), (10 ** 12 + 1, 10 ** 12 + 3 * 10 ** 6),
+                       ((1 << 32) - 10 ** 6, (1 << 32) + 10 ** 6), ((1 << 50) - 2 * 10 ** 6, (1 << 50) + 10 ** 6)]:
+            say("derived vs direct", bases[:4], lo, hi)
+            a = auto.scan(lo, hi, near_T=1000)
+            b = off.scan(lo, hi, near_T=1000)
+            assert (a["primes"], a["tests"], a["near"], a["flt"], a["cks_all"], a["sumq_all"], a["nev"]) == \
+                (b["primes"], b["tests"], b["near"], b["flt"], b["cks_all"], b["sumq_all"], b["nev"]), (lo, hi)
+            assert a["events"] == b["events"]
+        rng = random.Random(9)
+        ranges = [(lo, lo + rng.randrange(1, 10 ** 6)) for lo in (rng.randrange(10 ** 6, 1 << 50) for _ in range(40))]
+        pa, ea = auto.scan_batch(ranges, near_T=1000)
+        pb, eb = off.scan_batch(ranges, near_T=1000)
+        assert [(x["tests"], x["cks_all"], x["sumq_all"]) for x in pa] == [(x["tests"], x["cks_all"], x["sumq_all"])
+                                                                           for x in pb]
+        assert ea == eb
+    finally:
+        auto.close()
+        off.close()
+
+
 def test_int32_path_is_base_2_only():
     e = wf.Engine(arith=1)
     try:
