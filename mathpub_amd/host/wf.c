@@ -1118,8 +1118,12 @@ static int cmd_dcheck(int argc, char **argv)
     snprintf(outbuf, sizeof outbuf, "%s.dcheck", logp);
     const char *outp = npos > 4 ? pos[4] : outbuf;
     const long pid = npos > 5 ? (long)parse_num(pos[5]) : 0;
+    /* the recomputation takes paths a scan does not: prove mode, small segments, one ladder
+     * per base (a scan derives composite bases from their prime factors), and for base 2
+     * alone the 32-bit limb ladders */
     g_cfg.prove = WF_PROVE_ON;
     g_cfg.seg_bytes = 16384;
+    g_cfg.derive = WF_DERIVE_OFF;
     if (g_nb == 1 && g_b[0] == 2) g_cfg.arith = 1;
     engine_open();
     FILE *lf = fopen(logp, "r");
