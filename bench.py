@@ -134,6 +134,14 @@ def bench_c3(args):
     digest = 0
     for r in per:
         digest = (digest * 0x100000001B3 ^ r["cks"] ^ r["primes"]) & ((1 << 64) - 1)
+    # the oracle's per-range counts and checksums of the same 4096 ranges
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "c3_full.json")
+    if os.path.exists(gpath) and len(ranges) == 4096:
+        with open(gpath) as f:
+            g = json.load(f)
+        golden = ([r["primes"] for r in per] == g["per_range_primes"]
+                  and ["%016x" % r["cks"] for r in per] == g["per_range_cks2"])
     compare = None
     if args.c3_compare:
         # the same first N ranges through the plain single-range path (one wf_scan each)
@@ -155,7 +163,7 @@ def bench_c3(args):
                         ms, args, {"workload": "4096 ranges, starts U[1e12,1e18], lengths logU[1e6,1e9], seed "
                                    "0x243F6A8885A308D3", "engine": "batched", "ranges": len(ranges),
                                    "numbers": numbers, "tests": tests, "per_range_digest": "%016x" % digest,
-                                   "autotuned": tuned, "w_products_per_s": prods / (ms / 1e3),
+                                   "bit_exact_vs_golden": golden, "autotuned": tuned, "w_products_per_s": prods / (ms / 1e3),
                                    "plain_vs_batched_same_ranges": compare,
                                    "device_ms_per_step": sum(t["total_ms"] for t in tims) / len(tims),
                                    "host_plan_ms_per_step": sum(t["plan_ms"] for t in tims) / len(tims),
