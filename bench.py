@@ -25,6 +25,7 @@ METRIC = "primes Wieferich-tested/sec at p~1e15, batched vs plain GPU, bit-exact
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector peak (published), FMA = 2 flops
 FP64_PEAK_INSTR = 256 * 64 * 2.4e9   # wave64 FP64 VALU instructions: lanes per second
 INT32_MUL_PEAK = 256 * 32 * 2.4e9    # v_mad_u64_u32 products/s (half rate, measured)
+K_TEST_VALU_PER_TEST = 787           # k_test<A_F64> at [1e15, 1e15+1e11): profiles/c2_sq_counters_pmc.md
 
 
 def wf_products(pbits):
@@ -454,7 +455,11 @@ def main():
             roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_test<A_F64>",
                     "compute_pipe": "FP64 VALU (v_fma_f64); no MFMA: one modulus per lane",
-                    "valu_fp64_issue_frac": tests * instr / (test_ms / 1e3) / FP64_PEAK_INSTR}
+                    "valu_fp64_issue_frac": tests * instr / (test_ms / 1e3) / FP64_PEAK_INSTR,
+                    # all VALU instructions of the kernel (SQ_INSTS_VALU x 64 / tests, counted once
+                    # with rocprofv3 --pmc on this workload) against the same nominal-clock ceiling
+                    "valu_instr_per_test": K_TEST_VALU_PER_TEST,
+                    "valu_issue_frac": tests * K_TEST_VALU_PER_TEST / (test_ms / 1e3) / FP64_PEAK_INSTR}
         tj = os.path.join(ROOT, "profiles", "c2_k_test_traffic.json")
         if roof["kernel"] == "k_test<A_F64>" and span == SPAN and os.path.exists(tj):
             # HBM bytes per launch from the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
