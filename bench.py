@@ -378,7 +378,8 @@ def main():
                 "base3": bench_published, "prove": bench_prove, "stats": bench_stats}[args.workload](args)
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
-              "(ranges shard by splitting [lo, hi), see DESIGN.md)", file=sys.stderr)
+              "(the range would shard into contiguous sub-ranges, one engine per GPU, with the "
+              "totals summed mod 2^64 at the end)", file=sys.stderr)
         return 2
     from mathpub_amd import wf
     span = int(args.span)
