@@ -1,5 +1,7 @@
-"""GPU parity at the full sizes of the batched and top-of-word workloads.
+"""GPU parity at the full sizes of the single-range, batched and top-of-word workloads.
 
+* [1e15, 1e15 + 1e11): count, checksum, sum of q, near list and solutions against the oracle's
+  values for the whole range (tests/golden/c2_full.json), through wf_scan and wf_scan_batch.
 * 4096 batched ranges (starts uniform in [1e12, 1e18], lengths log-uniform in [1e6, 1e9]):
   every range's prime count and base-2 checksum, and the total sum of q, against the
   oracle's values for the same ranges (tests/golden/c3_full.json, make_c3_golden.py).
@@ -26,6 +28,21 @@ M64 = (1 << 64) - 1
 
 def say(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["fast", "int32"])
+def test_c2_full_vs_golden(arith):
+    g = json.load(open(os.path.join(G, "c2_full.json")))
+    say("C2 full [1e15, 1e15+1e11), arith", arith)
+    with wf.Engine(arith=arith) as e:
+        r = e.scan(g["lo"], g["hi"], near_T=g["near_T"])
+        per, ev = e.scan_batch([(g["lo"], g["hi"])], near_T=g["near_T"])
+    for x in (r, per[0]):
+        assert (x["primes"], x["tests"], x["near"], x["flt"]) == (g["primes"], g["tests"], g["near"], g["flt"])
+        assert "%016x" % x["cks"] == g["cks2"] and "%016x" % x["sumq"] == g["sumq2"]
+    near = [[p, q if q <= p // 2 else q - p] for t, p, q, rr, b in r["events"] if t == wf.EV_NEAR]
+    assert near == g["near_list"]
+    assert [p for t, p, q, rr, b in r["events"] if t == wf.EV_FOUND] == g["hits"]
 
 
 @pytest.mark.parametrize("arith", [0, 1], ids=["fast", "int32"])
