@@ -329,6 +329,9 @@ def bench_c5(args):
                     lat.append(time.perf_counter() - t0 - arrivals[i])
                     done_tests += r["tests"]
                     numbers += qs[i][1] - qs[i][0]
+                    if len(lat) % 2000 == 0:
+                        print("rate %g: %d queries done, %.1f s" % (rate, len(lat), time.perf_counter() - t0),
+                              file=sys.stderr, flush=True)
             time.sleep(0.0005)
         wall = time.perf_counter() - t0
         eng.serve_stop()
