@@ -115,7 +115,8 @@ def bench_c3(args):
     """4096 uneven ranges in one batched (HIP graph) call, per-range counts and checksums."""
     from mathpub_amd import wf
     ranges = c3_ranges()
-    eng = wf.Engine(arith=args.arith, prove=_prove(args))
+    eng = wf.Engine(arith=args.arith, prove=_prove(args), sieve_block=args.sieve_block, bucket_min=args.bucket_min,
+                    seg_bytes=args.seg_bytes)
     t = time.perf_counter()
     tuned = None if args.no_tune else eng.autotune_batch(ranges)
     print("autotune %s (%.1f s)" % (tuned, time.perf_counter() - t), file=sys.stderr, flush=True)
@@ -357,9 +358,9 @@ def main():
     ap.add_argument("--arith", type=int, default=0, help="1 = force the 32-bit Montgomery kernels")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
-    ap.add_argument("--seg-bytes", type=int, default=0, help="c2 with --no-tune: LDS segment bytes")
-    ap.add_argument("--sieve-block", type=int, default=0, help="c2 with --no-tune: threads per sieve workgroup")
-    ap.add_argument("--bucket-min", type=int, default=0, help="c2 with --no-tune: bucket threshold")
+    ap.add_argument("--seg-bytes", type=int, default=0, help="c2/c3 with --no-tune: LDS segment bytes")
+    ap.add_argument("--sieve-block", type=int, default=0, help="c2/c3 with --no-tune: threads per sieve workgroup")
+    ap.add_argument("--bucket-min", type=int, default=0, help="c2/c3 with --no-tune: bucket threshold")
     ap.add_argument("--span", type=float, default=SPAN, help="numbers per step (default 1e11)")
     ap.add_argument("--mode", choices=["both", "plain", "batched"], default="both")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "oeis", "base3", "prove", "stats"], default="c2",

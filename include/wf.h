@@ -23,7 +23,7 @@
  *   wf_sieve_stats the GPU sieve for the repository's other prime consumers: twin counts
  *                  (A007508/README.md:3-5, 44-64) and runs of increasing gaps
  *                  (A158939/a158939.c:19-24, A158939/cuda/a158939_cuda.cu:53-78)
- *   wf_serve_*     (no counterpart: a device-resident request loop)
+ *   wf_serve_*     (no counterpart: continuous batching of range queries)
  *
  * Conventions (as wieferich.c): return 0 on success or a negative WF_E_* code
  * (wf_strerror names it; the host program prints it and exits with status 1, as die()
@@ -65,6 +65,9 @@ typedef struct {
     uint32_t type, j;       /* type, base index */
     uint64_t p, q;          /* prime and Fermat quotient (q >= p for FLT) */
     uint64_t r_hi, r_lo;    /* b^(p-1) mod p^2 */
+    uint32_t range;         /* index of the range in wf_scan_batch's list (0 for wf_scan and
+                               for the events of a served request) */
+    uint32_t pad;
 } wf_event;
 
 /* Totals of one scanned range: the reference's slot_t (wieferich.c:956-960) plus the
@@ -97,6 +100,11 @@ typedef struct {
     uint32_t derive;        /* composite bases from their prime factors' quotients (p < 2^50):
                                WF_DERIVE_AUTO (whenever that needs fewer ladders than bases),
                                WF_DERIVE_OFF (one ladder per base) */
+    uint32_t fault_step;    /* testing only, 0 = off: serving step number fault_step (counted
+                               from 1) fails with WF_E_DEVICE instead of being enqueued */
+    uint64_t fault_p;       /* testing only, 0 = off: the residue of this p is taken as r + 1,
+                               which fails the FLT check (exercises the FLT-ERROR path: the
+                               reference's finish_lanes, wieferich.c:1058-1063) */
 } wf_config;
 
 /* Prove mode: a piece is sieved by the primes below bucket_min only, and its survivors are
@@ -132,7 +140,8 @@ int  wf_scan(wf_engine *e, uint64_t lo, uint64_t hi, int64_t near_T, wf_result *
              wf_event *ev, uint64_t ev_cap, int want_psp, uint64_t *psp, uint64_t psp_cap);
 
 /* Scan n ranges in one device pass: per_range[i] receives range i's totals; events of
- * all ranges go to ev (sorted by (p, j); a range is found by p). */
+ * all ranges go to ev, sorted by (p, j, range), each with the index of its range (ranges may
+ * overlap or repeat: an event of a number in two ranges appears once per range). */
 int  wf_scan_batch(wf_engine *e, const wf_range *r, uint32_t n, int64_t near_T,
                    wf_result *per_range, wf_event *ev, uint64_t ev_cap);
 
@@ -168,14 +177,24 @@ int  wf_sieve_stats(wf_engine *e, uint64_t lo, uint64_t hi, wf_stats *out);
 /* BPSW verdicts of n[0..count): flags[i] bit 0 = base-2 strong probable prime, bit 1 =
  * strong Lucas probable prime with Selfridge's parameters (both for odd n >= 3), bit 2 =
  * prime (exact below 2^64).  Any n is accepted (even n and n < 3 give 0, or 4 for n = 2). */
+enum { WF_BPSW_SPRP2 = 1, WF_BPSW_LUCAS = 2, WF_BPSW_PRIME = 4 };
 int  wf_bpsw(wf_engine *e, const uint64_t *n, uint8_t *flags, uint64_t count);
 
-/* Serving: a device-resident request loop with continuous batching.  submit admits a
- * range when a slot and the HBM budget allow (else WF_E_BUSY); poll returns its result
- * once done (*done = 1).  Results equal wf_scan of the same range. */
+/* Serving: continuous batching over the device-resident sieving-prime table.  A server
+ * thread owns the engine between wf_serve_start and wf_serve_stop.  It runs steps: each takes
+ * the next piece of every admitted request and sieves and tests all of them in the same
+ * kernel launches.  It plans and enqueues step k + 1 while step k runs on the GPU, so requests
+ * that arrive meanwhile join the next step.
+ * submit admits a range with its near-hit bound when a slot and the HBM budget allow (else
+ * WF_E_BUSY: retry later).  poll returns its totals once done (*done = 1), with its FOUND /
+ * NEAR / FLT events sorted by (p, j) into ev[0..ev_cap) (out->nev counts all of them).  The
+ * results equal wf_scan's of the same range and near_T.  A request whose events did not all
+ * fit the step's device event buffer is returned with WF_E_OVERFLOW (and *done = 1); it is
+ * never truncated silently.  A failed step ends every request with its error code, and the
+ * server takes no more until wf_serve_stop. */
 int  wf_serve_start(wf_engine *e, uint32_t slots, uint64_t hbm_budget);
-int  wf_serve_submit(wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *ticket);
-int  wf_serve_poll(wf_engine *e, uint64_t ticket, wf_result *out, int *done);
+int  wf_serve_submit(wf_engine *e, uint64_t lo, uint64_t hi, int64_t near_T, uint64_t *ticket);
+int  wf_serve_poll(wf_engine *e, uint64_t ticket, wf_result *out, wf_event *ev, uint64_t ev_cap, int *done);
 int  wf_serve_stop(wf_engine *e);
 
 /* Autotuner: times the sieve stage on a sample of [lo, hi) over segment sizes 16-128 KiB,

@@ -38,12 +38,14 @@ def build_probe():
     run([HIPCC] + flags + ["-o", os.path.join(PKG, "wf_probe"), src])
 
 
-def build_host():
-    src = os.path.join(PKG, "host", "wf.c")
-    if not os.path.exists(src):
-        return
-    run(["gcc", "-O2", "-std=gnu11", "-Wall", "-Wextra", "-pthread", "-I" + INC, "-o", HOST, src,
-         "-L" + PKG, "-lwf", "-Wl,-rpath,$ORIGIN", "-lm"])
+HOST_SRC = [os.path.join(PKG, "host", f) for f in ("wf_main.c", "wf_text.c")]
+
+
+def build_host(out=HOST, extra=(), rpath="$ORIGIN"):
+    """The wf command (C, linked against libwf.so, found through rpath); extra = additional
+    gcc flags (the sanitizer test builds a copy with -fsanitize=address,undefined)."""
+    run(["gcc", "-O2", "-std=gnu11", "-Wall", "-Wextra", "-pthread", "-I" + INC] + list(extra) + ["-o", out] + HOST_SRC +
+        ["-L" + PKG, "-lwf", "-Wl,-rpath," + rpath, "-lm"])
 
 
 def main():

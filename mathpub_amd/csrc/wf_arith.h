@@ -268,7 +268,7 @@ __device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64
   m1.ip = 1.0 / pd1;
   const int nw = (ebits + W - 1) / W;
   int sh = (nw - 1) * W;
-  const u32 c0 = pw0[(int)(e0 >> sh)], c1 = pw1[(int)(e1 >> sh)];
+  const u32 c0 = pw0[(int)(e0 >> sh) & ((1 << W) - 1)], c1 = pw1[(int)(e1 >> sh) & ((1 << W) - 1)];
   a0 = 1.0; b0 = 0.0; a1 = 1.0; b1 = 0.0;
   if ((double)c0 < pd0) a0 = (double)c0;
   else f64_mul_small(a0, b0, (double)c0, m0);
@@ -640,7 +640,9 @@ __device__ __forceinline__ void m28_powc(u64 p, u64 e, int ebits, const u32* pw,
   for (int i = 0; i < 5; i++) x[i] = (u32)(m.rmodn >> (28 * i)) & M28;
   const int nw = (ebits + W - 1) / W;
   int sh = (nw - 1) * W;
-  m28_mul_small(x, pw[(int)(e >> sh)], m);
+  // e < 2^ebits for every active lane, so the leading digit is below 2^W; the mask keeps an
+  // idle lane's longer exponent inside the table as well
+  m28_mul_small(x, pw[(int)(e >> sh) & ((1 << W) - 1)], m);
   for (sh -= W; sh >= 0; sh -= W) {
 #pragma unroll
     for (int s = 0; s < W; s++) m28_sqr(x, m.n, m.ninv);

@@ -674,7 +674,9 @@ struct TestOut {
   DevEvent* ev;
   u64 ev_cap;
   unsigned long long* nev;   // total events (device counter)
-  i64 near_T;
+  i64 near_T;                // near-hit bound of every range, unless nearv is set
+  const i64* nearv;          // per range (serving: per slot), or nullptr
+  u64 fault_p;               // testing only: the residue of this p is taken as r + 1 (0 = off)
   u32 nb;                    // bases of the run
 };
 
@@ -709,7 +711,12 @@ __device__ __forceinline__ bool near_hit(u64 p, u64 q, i64 T) {
 // and add the lane's terms to its running sums.  r = r_hi:r_lo, q = (r - 1) p^-1 mod 2^64.
 struct LaneSums { u32 tests, flt, near; u64 cks, sumq; };   // counts per lane and launch fit 32 bits
 
-__device__ __forceinline__ void account(const TestOut& o, u32 rid, u32 j, u64 p, u64 rhi, u64 rlo, LaneSums& s) {
+__device__ __forceinline__ void account(const TestOut& o, u32 rid, u32 j, u64 p, u64 rhi, u64 rlo, LaneSums& s,
+                                        i64 nT) {
+  if (p == o.fault_p) {      // injected fault: r + 1 is not 1 mod p, so the FLT check must catch it
+    rlo += 1;
+    rhi += rlo == 0 ? 1 : 0;
+  }
   const u64 pinv = inv64(p);
   const u64 q = (rlo - 1) * pinv;
   s.tests++;
@@ -722,7 +729,7 @@ __device__ __forceinline__ void account(const TestOut& o, u32 rid, u32 j, u64 p,
   s.sumq += q;
   if (q == 0) {
     push_event(o, WF_EV_FOUND, rid, j, p, q, rhi, rlo);
-  } else if (o.near_T > 0 && near_hit(p, q, o.near_T)) {
+  } else if (nT > 0 && near_hit(p, q, nT)) {
     s.near++;
     push_event(o, WF_EV_NEAR, rid, j, p, q, rhi, rlo);
   }
@@ -732,15 +739,15 @@ __device__ __forceinline__ void account(const TestOut& o, u32 rid, u32 j, u64 p,
 // not dividing the base, r = 1 (mod p), so a = 1 and q = b mod p (f64_quotient); p^-1 and r
 // are only formed for an event or a failed FLT check.
 __device__ __forceinline__ void account_padic(const TestOut& o, u32 rid, u32 j, u64 p, double pd, double a,
-                                              double b, LaneSums& s) {
+                                              double b, LaneSums& s, i64 nT) {
   u64 q;
-  if (f64_quotient(a, b, pd, q)) {             // q < p: the FLT check holds
+  if (p != o.fault_p && f64_quotient(a, b, pd, q)) {   // q < p: the FLT check holds
     s.tests++;
     s.cks += d_mix64(q ^ (p * GOLD));
     s.sumq += q;
     if (q == 0) {
       push_event(o, WF_EV_FOUND, rid, j, p, 0, 0, 1);
-    } else if (o.near_T > 0 && near_hit(p, q, o.near_T)) {
+    } else if (nT > 0 && near_hit(p, q, nT)) {
       s.near++;
       const u64 lo = q * p, hi = __umul64hi(q, p);
       const u64 rlo = lo + 1;
@@ -754,7 +761,7 @@ __device__ __forceinline__ void account_padic(const TestOut& o, u32 rid, u32 j, 
   f64_normalise(a, b, m);
   u64 rhi, rlo;
   padic_to_r(p, a, b, rhi, rlo);
-  account(o, rid, j, p, rhi, rlo, s);
+  account(o, rid, j, p, rhi, rlo, s, nT);
 }
 
 // Wavefront sums of a (range, base)'s lane totals -> one atomic per counter (lane 0).
@@ -866,6 +873,7 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
     const u32 b = c_base[j];
     const int w = (int)c_bwin[j];
     const u32* pw = pw_all + j * BPOW;
+    const i64 nT = o.nearv ? o.nearv[d.rid] : o.near_T;
     const u32* po = off + (u64)sg * pcap;
     // exponent length of the segment's largest number: no prime of the tile has a longer
     // one, so the ladders of a wavefront agree without a cross-lane reduction
@@ -906,8 +914,8 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
             default: fp64_powc_x2<1>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
           }
         }
-        if (act0 && !divides_base(p0, b)) account_padic(o, d.rid, j, p0, pd0, a0, b0, s);
-        if (act1 && !divides_base(p1, b)) account_padic(o, d.rid, j, p1, pd1, a1, b1, s);
+        if (act0 && !divides_base(p0, b)) account_padic(o, d.rid, j, p0, pd0, a0, b0, s, nT);
+        if (act1 && !divides_base(p1, b)) account_padic(o, d.rid, j, p1, pd1, a1, b1, s, nT);
       }
     } else {
       for (u32 base = beg; base < end; base += blockDim.x) {
@@ -916,7 +924,7 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
         const u64 p = active ? d.base + po[i] : idle;
         u64 rhi, rlo;
         residue<ARITH>(p, eb, b, w, pw, rhi, rlo);
-        if (active && !divides_base(p, b)) account(o, d.rid, j, p, rhi, rlo, s);
+        if (active && !divides_base(p, b)) account(o, d.rid, j, p, rhi, rlo, s, nT);
       }
     }
   }
@@ -978,6 +986,7 @@ __global__ void __launch_bounds__(256) k_test_derived(const SegDesc* __restrict_
     const int eb = etop ? 64 - __builtin_clzll(etop) : 1;
     const bool big = d.base >= (1ull << 32);
     const u64 idle = big ? 4294967311ull : 3ull;
+    const i64 nT = o.nearv ? o.nearv[d.rid] : o.near_T;
     const double based = (double)d.base;          // p = base + offset < 2^50: exact as a double
     for (u32 base = beg; base < end; base += blockDim.x) {
       const u32 i = base + threadIdx.x;
@@ -999,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_test_derived(const SegDesc* __restrict_
         LaneSums ls = {0, 0, 0, 0, 0};
         if (act && !divides_base(p, b)) {
           u64 q = 0;
-          bool ok = true;
+          bool ok = p != o.fault_p;         // an injected fault takes the direct ladder
           for (u32 sl = 0; sl < nl && ok; sl++) {
             const u32 ex = c_lexp[j * WF_MAXL + sl];
             if (!ex) continue;
@@ -1016,7 +1025,7 @@ __global__ void __launch_bounds__(256) k_test_derived(const SegDesc* __restrict_
             ls.sumq = q;
             if (q == 0) {
               push_event(o, WF_EV_FOUND, cur, j, p, 0, 0, 1);
-            } else if (o.near_T > 0 && near_hit(p, q, o.near_T)) {
+            } else if (nT > 0 && near_hit(p, q, nT)) {
               ls.near = 1;
               const u64 lo = q * p, hi = __umul64hi(q, p);
               const u64 rlo = lo + 1;
@@ -1025,7 +1034,7 @@ __global__ void __launch_bounds__(256) k_test_derived(const SegDesc* __restrict_
           } else {
             u64 rhi, rlo;
             residue<A_F64>(p, eb, b, (int)c_bwin[j], c_bpow + j * BPOW, rhi, rlo);
-            account(o, cur, j, p, rhi, rlo, ls);
+            account(o, cur, j, p, rhi, rlo, ls, nT);
           }
         }
         const u64 c = wave_sum(ls.cks), sq = wave_sum(ls.sumq);
@@ -2203,10 +2212,35 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
   return WF_OK;
 }
 
+// Events in (p, base, range) order: the same for any schedule of the kernels.
 static bool cmp_dev_event(const DevEvent& a, const DevEvent& b) {
   if (a.p != b.p) return a.p < b.p;
   if (a.j != b.j) return a.j < b.j;
+  if (a.rid != b.rid) return a.rid < b.rid;
   return a.type < b.type;
+}
+
+static void to_event(const DevEvent& d, u32 range, wf_event& o) {
+  o.type = d.type;
+  o.j = d.j;
+  o.p = d.p;
+  o.q = d.q;
+  o.r_hi = d.r_hi;
+  o.r_lo = d.r_lo;
+  o.range = range;
+  o.pad = 0;
+}
+
+// The p = 2 solutions of a range containing 2: one for every base b = 1 (mod 4), which
+// scan_chunk records as a hit (wieferich.c:1143-1146); 2 itself is counted, never tested.
+static bool has_two(u64 lo, u64 hi) { return lo <= 2 && (hi == 0 || hi > 2); }
+static void push_two_events(const wf_engine* e, u32 rid, std::vector<DevEvent>& out) {
+  for (int j = 0; j < e->nb; j++)
+    if (e->bases[j] % 4 == 1) {
+      DevEvent x;
+      x.type = WF_EV_FOUND; x.rid = rid; x.j = (u32)j; x.pad = 0; x.p = 2; x.q = 0; x.r_hi = 0; x.r_lo = 1;
+      out.push_back(x);
+    }
 }
 
 // Scan n ranges; per-range totals into res[0..n), events into ev.
@@ -2246,6 +2280,8 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   to.ev_cap = e->dev_ev_cap;
   to.nev = e->d_ctr + 1;
   to.near_T = near_T;
+  to.nearv = nullptr;
+  to.fault_p = e->cfg.fault_p;
   memset(&e->last, 0, sizeof e->last);
   // pieces by arithmetic class, then passes of at most cfg.chunk numbers each
   std::vector<Piece> all;
@@ -2303,17 +2339,12 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   const u64 got = std::min<u64>(nev, e->dev_ev_cap);
   std::vector<DevEvent> dev(got);
   if (got) HIPCHK(hipMemcpy(dev.data(), e->d_ev, got * sizeof(DevEvent), hipMemcpyDeviceToHost));
-  // p = 2 is counted, never tested; it is a solution for every base b = 1 (mod 4), which
-  // scan_chunk records as a hit (wieferich.c:1143-1146)
   for (u32 i = 0; i < n; i++)
-    if (r[i].lo <= 2 && (r[i].hi == 0 || r[i].hi > 2))
-      for (int j = 0; j < e->nb; j++)
-        if (e->bases[j] % 4 == 1) {
-          DevEvent x;
-          x.type = WF_EV_FOUND; x.rid = i; x.j = (u32)j; x.pad = 0; x.p = 2; x.q = 0; x.r_hi = 0; x.r_lo = 1;
-          dev.push_back(x);
-          acc[i].nev++;
-        }
+    if (has_two(r[i].lo, r[i].hi)) {
+      const size_t before = dev.size();
+      push_two_events(e, i, dev);
+      acc[i].nev += dev.size() - before;
+    }
   std::sort(dev.begin(), dev.end(), cmp_dev_event);
   const u64 ndev = dev.size();
   u64 tested = 0;
@@ -2322,7 +2353,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
     memset(&o, 0, sizeof o);
     o.primes = acc[i].primes;
     // the prime 2 is counted, never tested (wieferich.c:1142-1146)
-    if (r[i].lo <= 2 && (r[i].hi == 0 || r[i].hi > 2)) o.primes++;
+    if (has_two(r[i].lo, r[i].hi)) o.primes++;
     o.tests = acc[i].tests;
     o.near = acc[i].near;
     o.flt = acc[i].flt;
@@ -2333,14 +2364,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
     o.nev = acc[i].nev;
     tested += o.tests;
   }
-  for (u64 k = 0; k < ndev && k < ev_cap; k++) {
-    ev[k].type = dev[k].type;
-    ev[k].j = dev[k].j;
-    ev[k].p = dev[k].p;
-    ev[k].q = dev[k].q;
-    ev[k].r_hi = dev[k].r_hi;
-    ev[k].r_lo = dev[k].r_lo;
-  }
+  for (u64 k = 0; k < ndev && k < ev_cap; k++) to_event(dev[k], dev[k].rid, ev[k]);
   e->last.tested = tested;
   if (!batched) e->last.total_ms = e->last.sieve_ms + e->last.test_ms;
   for (u32 i = 0; i < n; i++) {
@@ -2639,21 +2663,77 @@ extern "C" int wf_last_timing(wf_engine* e, wf_timing* t) {
 }
 
 // ---------------------------------------------------------------------------------
-// serving: a device-resident request loop with continuous batching
+// serving: continuous batching of range queries
 // ---------------------------------------------------------------------------------
 //
-// Admitted requests own a slot (a row of per-request accumulators in HBM).  The server
-// thread runs steps: each step takes the next piece of every active request (at least
-// min_piece numbers, at most an even share of the step budget), sieves and tests all of
-// them in the same kernel launches (range id = slot), and retires the requests whose last
-// piece is done.  Requests that arrive while a step runs join the next step.  The step
-// budget follows from the HBM budget and the bytes a pass needs per number.
+// An admitted request owns a slot: a row of accumulators, checksums and its near-hit bound in
+// HBM.  The server thread runs steps.  A step takes the next piece of every request that still
+// has numbers to schedule (at least min_piece numbers, at most an even share of the step
+// budget), one pass per arithmetic class, and sieves and tests all pieces of a pass in the
+// same launches (range id = slot).  Two steps are in flight at once: while step k runs, the
+// thread plans and enqueues step k + 1, which takes the requests that arrived meanwhile.
+// Passes alternate between the engine's two buffer sets as in the batched path: a pass
+// sieves on the engine's stream while the previous pass's tests run on the second stream.
+// Each step has its own event buffer and copies the slots' totals to pinned host memory after
+// its last test; the thread then routes the step's events to their requests by slot, and a
+// request whose last piece was in the step is finished.  A slot handed to a new request is
+// cleared by a kernel on the engine's stream before the step that first uses it, so no
+// memset runs while steps are in flight.
+
+#define SERVE_EVCAP (1u << 18)      // device events per step
+#define SERVE_INIT_MAX 32           // slots set up per k_slot_init launch
+
+struct SlotInit {
+  u32 n, pad;
+  u32 slot[SERVE_INIT_MAX];
+  i64 near_T[SERVE_INIT_MAX];
+};
+
+// Clear the accumulators of newly admitted slots and set their near-hit bounds.
+__global__ void k_slot_init(DevAcc* __restrict__ acc, unsigned long long* __restrict__ cks,
+                            unsigned long long* __restrict__ sumq, i64* __restrict__ nearv, u32 nb, SlotInit s) {
+  for (u32 k = threadIdx.x; k < s.n; k += blockDim.x) {
+    const u32 sl = s.slot[k];
+    DevAcc z = {0, 0, 0, 0, 0, 0, 0};
+    acc[sl] = z;
+    nearv[sl] = s.near_T[k];
+    for (u32 j = 0; j < nb; j++) {
+      cks[(u64)sl * nb + j] = 0;
+      sumq[(u64)sl * nb + j] = 0;
+    }
+  }
+}
 
 struct ServeReq {
-  u64 lo, hi, next;   // hi == 0: 2^64; next = first number not yet scheduled
+  u64 lo, hi, next;        // hi == 0: 2^64; next = first number not yet scheduled
+  i64 near_T;
   u32 slot;
+  bool scheduled;          // every number is in a step
   bool done;
+  int rc;                  // WF_OK, or the error the request ended with
   wf_result res;
+  std::vector<DevEvent> ev;   // its events so far
+};
+
+struct ServeStep {         // a step in flight
+  std::vector<PassPlan> plans;   // kept until the step is done
+  std::vector<int> arith;
+  std::vector<u64> finishing;    // tickets whose last piece is in this step
+  u32 nslot = 0;                 // slots copied back (highest slot in use + 1)
+  SegDesc* d_segs = nullptr;
+  u64 seg_cap = 0;
+  FillRun* d_runs = nullptr;
+  u64 run_cap = 0;
+  FillDesc* d_fills = nullptr;
+  u64 fill_cap = 0;
+  DevEvent* d_ev = nullptr;
+  unsigned long long* d_nev = nullptr;
+  DevAcc* h_acc = nullptr;       // pinned
+  unsigned long long* h_cks = nullptr;
+  unsigned long long* h_sumq = nullptr;
+  unsigned long long* h_nev = nullptr;
+  u32* h_flags = nullptr;
+  hipEvent_t done = nullptr;
 };
 
 struct Server {
@@ -2662,103 +2742,293 @@ struct Server {
   std::condition_variable cv;
   bool stop = false;
   int err = WF_OK;
-  u32 slots = 0, active = 0;
+  u32 slots = 0, active = 0, nb = 1;
   u64 step_numbers = 0, min_piece = 0;
-  std::vector<u64> slot_ticket;   // 0 = free
+  std::vector<u64> slot_ticket;    // 0 = free
+  std::vector<std::pair<u32, i64>> fresh;   // slots admitted since the last step, with near_T
   std::unordered_map<u64, ServeReq> reqs;
   u64 next_ticket = 1;
-  DevAcc* d_acc = nullptr;        // per slot
+  DevAcc* d_acc = nullptr;         // per slot
   unsigned long long* d_cks = nullptr;    // per slot and base
   unsigned long long* d_sumq = nullptr;
-  u32 nb = 1;
-  u64 steps = 0;
+  i64* d_nearv = nullptr;          // per slot
+  ServeStep step[2];
+  std::vector<int> inflight;       // step contexts in flight, oldest first
+  int next_ctx = 0;
+  u64 gpass = 0;                   // passes enqueued so far (their buffer sets alternate)
+  u64 planned = 0;                 // steps planned
+  u64 steps = 0;                   // steps finished
 };
+
+// Wait for the oldest step in flight and hand its results to the requests.
+static int serve_finish_oldest(wf_engine* e, Server* S) {
+  ServeStep& c = S->step[S->inflight.front()];
+  S->inflight.erase(S->inflight.begin());
+  if (hipEventSynchronize(c.done) != hipSuccess) return WF_E_DEVICE;
+  if (*c.h_flags) return WF_E_OVERFLOW;   // a sieve bound was exceeded (bug guard)
+  const u64 nev = *c.h_nev;
+  std::vector<DevEvent> ev(std::min<u64>(nev, SERVE_EVCAP));
+  if (!ev.empty() && hipMemcpy(ev.data(), c.d_ev, ev.size() * sizeof(DevEvent), hipMemcpyDeviceToHost) != hipSuccess)
+    return WF_E_DEVICE;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->steps++;
+  for (const DevEvent& x : ev) {
+    const u64 t = x.rid < S->slots ? S->slot_ticket[x.rid] : 0;
+    auto it = S->reqs.find(t);
+    if (it != S->reqs.end()) it->second.ev.push_back(x);
+  }
+  for (u64 t : c.finishing) {
+    auto it = S->reqs.find(t);
+    if (it == S->reqs.end()) continue;
+    ServeReq& q = it->second;
+    const u32 k = q.slot;
+    const DevAcc& a = c.h_acc[k];
+    memset(&q.res, 0, sizeof q.res);
+    q.res.primes = a.primes;
+    q.res.tests = a.tests;
+    q.res.near = a.near;
+    q.res.flt = a.flt;
+    for (u32 j = 0; j < S->nb; j++) {
+      q.res.cks[j] = c.h_cks[(size_t)k * S->nb + j];
+      q.res.sumq[j] = c.h_sumq[(size_t)k * S->nb + j];
+    }
+    q.res.nev = a.nev;
+    // every event the kernels counted must have reached the step's buffer
+    q.rc = q.ev.size() == a.nev ? WF_OK : WF_E_OVERFLOW;
+    if (has_two(q.lo, q.hi)) {       // 2 is counted, never tested; its solutions as in wf_scan
+      q.res.primes++;
+      const size_t before = q.ev.size();
+      push_two_events(e, k, q.ev);
+      q.res.nev += q.ev.size() - before;
+    }
+    std::sort(q.ev.begin(), q.ev.end(), cmp_dev_event);
+    q.done = true;
+    S->slot_ticket[k] = 0;
+    S->active--;
+  }
+  S->cv.notify_all();
+  return WF_OK;
+}
+
+static int serve_drain(wf_engine* e, Server* S) {
+  int rc = WF_OK;
+  while (!S->inflight.empty() && rc == WF_OK) rc = serve_finish_oldest(e, S);
+  return rc;
+}
+
+// A failed step ends every request with its error; the server then takes no more.
+static void serve_fail(wf_engine* e, Server* S, int rc) {
+  (void)hipStreamSynchronize(e->st);
+  (void)hipStreamSynchronize(e->st2);
+  S->inflight.clear();
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->err = rc;
+  for (auto& kv : S->reqs)
+    if (!kv.second.done) {
+      kv.second.done = true;
+      kv.second.rc = rc;
+    }
+  std::fill(S->slot_ticket.begin(), S->slot_ticket.end(), 0);
+  S->fresh.clear();
+  S->active = 0;
+  S->cv.notify_all();
+}
+
+static bool set_fits(const BufSet& B, const PassPlan& P) {
+  const u64 nseg = P.segs.size();
+  return B.cnt_cap >= nseg && B.off_cap >= nseg * P.pcap && (!P.bcap || (B.bcnt_cap >= nseg &&
+         B.bdata_cap >= nseg * (u64)P.bcap)) && (!P.nprove || B.tcnt_cap >= nseg * (u64)prove_tiles(P));
+}
+
+// Plan the passes of one step's pieces and enqueue them into context c.
+static int serve_enqueue(wf_engine* e, Server* S, int ci, const std::vector<Piece>& all,
+                         const std::vector<std::pair<u32, i64>>& inits) {
+  ServeStep& c = S->step[ci];
+  int rc;
+  u64 maxtop = 0;
+  for (const Piece& p : all) maxtop = std::max<u64>(maxtop, p.hi ? p.hi - 1 : 0xFFFFFFFFFFFFFFFFull);
+  const u64 need = std::min<u64>(std::max<u64>(isqrt_u64(maxtop) + 1, 64), 0xFFFFFFFFull);
+  if (need > e->sp_lim) {            // the table is rebuilt: nothing in flight may read it
+    if ((rc = serve_drain(e, S)) != WF_OK) return rc;
+    if ((rc = ensure_sieving_primes(e, need)) != WF_OK) return rc;
+  }
+  c.plans.clear();
+  c.arith.clear();
+  for (int cls = A_F64; cls < A_NCLS; cls++) {
+    std::vector<Piece> ps;
+    for (const Piece& p : all)
+      if (p.arith == cls) ps.push_back(p);
+    if (ps.empty()) continue;
+    c.plans.emplace_back();
+    plan_pass(e, ps, c.plans.back());
+    c.arith.push_back(cls);
+  }
+  u64 nsegs = 0, nruns = 0, nfills = 0;
+  for (const PassPlan& P : c.plans) {
+    nsegs += P.segs.size();
+    nruns += P.fruns.size();
+    nfills += P.fills.size();
+  }
+  if ((rc = grow(c.d_segs, c.seg_cap, std::max<u64>(nsegs, 1))) != WF_OK) return rc;
+  if ((rc = grow(c.d_runs, c.run_cap, std::max<u64>(nruns, 1))) != WF_OK) return rc;
+  if ((rc = grow(c.d_fills, c.fill_cap, std::max<u64>(nfills, 1))) != WF_OK) return rc;
+  // buffer sets: a pass that needs more room than its set has waits for everything in flight
+  for (size_t k = 0; k < c.plans.size(); k++) {
+    BufSet& B = e->bs[(S->gpass + k) & 1];
+    if (!set_fits(B, c.plans[k])) {
+      if ((rc = serve_drain(e, S)) != WF_OK) return rc;
+      if ((rc = size_set(B, c.plans[k])) != WF_OK) return rc;
+    }
+  }
+  // new slots, the step's event counter, then the descriptors
+  for (size_t k = 0; k < inits.size(); k += SERVE_INIT_MAX) {
+    SlotInit si;
+    memset(&si, 0, sizeof si);
+    si.n = (u32)std::min<size_t>(SERVE_INIT_MAX, inits.size() - k);
+    for (u32 i = 0; i < si.n; i++) {
+      si.slot[i] = inits[k + i].first;
+      si.near_T[i] = inits[k + i].second;
+    }
+    hipLaunchKernelGGL(k_slot_init, dim3(1), dim3(64), 0, e->st, S->d_acc, S->d_cks, S->d_sumq, S->d_nearv, S->nb, si);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, e->st, (u32*)c.d_nev, (u64)2);
+  HIPCHK(hipGetLastError());
+  u64 so = 0, ro = 0, fo = 0;
+  std::vector<u64> soff, roff, foff;
+  for (const PassPlan& P : c.plans) {
+    soff.push_back(so);
+    roff.push_back(ro);
+    foff.push_back(fo);
+    HIPCHK(hipMemcpyAsync(c.d_segs + so, P.segs.data(), P.segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, e->st));
+    if (!P.fruns.empty())
+      HIPCHK(hipMemcpyAsync(c.d_runs + ro, P.fruns.data(), P.fruns.size() * sizeof(FillRun), hipMemcpyHostToDevice, e->st));
+    if (!P.fills.empty())
+      HIPCHK(hipMemcpyAsync(c.d_fills + fo, P.fills.data(), P.fills.size() * sizeof(FillDesc), hipMemcpyHostToDevice,
+                            e->st));
+    so += P.segs.size();
+    ro += P.fruns.size();
+    fo += P.fills.size();
+  }
+  TestOut to;
+  to.acc = S->d_acc;
+  to.cks = S->d_cks;
+  to.sumq = S->d_sumq;
+  to.nb = S->nb;
+  to.ev = c.d_ev;
+  to.ev_cap = SERVE_EVCAP;
+  to.nev = c.d_nev;
+  to.near_T = 0;
+  to.nearv = S->d_nearv;
+  to.fault_p = e->cfg.fault_p;
+  for (size_t k = 0; k < c.plans.size(); k++) {
+    const PassPlan& P = c.plans[k];
+    const int set = (int)((S->gpass + k) & 1);
+    hipEvent_t sieved = e->gev[1 + set], tested = e->gev[3 + set];
+    if (S->gpass + k >= 2) HIPCHK(hipStreamWaitEvent(e->st, tested, 0));
+    if ((rc = enqueue_sieve(e, P, c.d_segs + soff[k], c.d_runs + roff[k], c.d_fills + foff[k], e->bs[set], e->st,
+                            S->d_acc)) != WF_OK)
+      return rc;
+    HIPCHK(hipEventRecord(sieved, e->st));
+    HIPCHK(hipStreamWaitEvent(e->st2, sieved, 0));
+    if ((rc = enqueue_test(e, P, c.d_segs + soff[k], e->bs[set], e->st2, c.arith[k], to)) != WF_OK) return rc;
+    HIPCHK(hipEventRecord(tested, e->st2));
+  }
+  S->gpass += c.plans.size();
+  // the slots' totals after this step (a request finishing here has no piece in a later one)
+  const size_t nck = (size_t)c.nslot * S->nb * sizeof(unsigned long long);
+  HIPCHK(hipMemcpyAsync(c.h_acc, S->d_acc, (size_t)c.nslot * sizeof(DevAcc), hipMemcpyDeviceToHost, e->st2));
+  HIPCHK(hipMemcpyAsync(c.h_cks, S->d_cks, nck, hipMemcpyDeviceToHost, e->st2));
+  HIPCHK(hipMemcpyAsync(c.h_sumq, S->d_sumq, nck, hipMemcpyDeviceToHost, e->st2));
+  HIPCHK(hipMemcpyAsync(c.h_nev, c.d_nev, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st2));
+  HIPCHK(hipMemcpyAsync(c.h_flags, e->d_flags, sizeof(u32), hipMemcpyDeviceToHost, e->st2));
+  HIPCHK(hipEventRecord(c.done, e->st2));
+  S->inflight.push_back(ci);
+  return WF_OK;
+}
 
 static void serve_loop(wf_engine* e) {
   Server* S = e->srv;
   (void)hipSetDevice(e->dev);
   for (;;) {
-    std::vector<std::pair<u64, u64>> sched;     // (ticket, new next)
     std::vector<Piece> all;
+    std::vector<std::pair<u32, i64>> inits;
+    int ci = -1;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv.wait(lk, [&] { return S->stop || S->active > 0; });
-      if (S->stop) return;
-      const u64 share = std::max<u64>(S->min_piece, S->step_numbers / std::max<u32>(S->active, 1));
-      for (u32 k = 0; k < S->slots; k++) {
-        const u64 t = S->slot_ticket[k];
-        if (!t) continue;
-        ServeReq& q = S->reqs[t];
-        if (q.done) continue;
-        const double rem = span_d(q.next, q.hi);
-        const bool last = rem <= (double)share;
-        const u64 end = last ? q.hi : q.next + share;
-        split_classes(e, q.next, end, k, all);
-        sched.push_back({t, end});
+      auto schedulable = [&]() {
+        for (u32 k = 0; k < S->slots; k++) {
+          const u64 t = S->slot_ticket[k];
+          if (t && !S->reqs[t].scheduled) return true;
+        }
+        return false;
+      };
+      S->cv.wait(lk, [&] { return S->stop || !S->inflight.empty() || (S->err == WF_OK && schedulable()); });
+      if (S->stop) {
+        lk.unlock();
+        (void)hipStreamSynchronize(e->st);
+        (void)hipStreamSynchronize(e->st2);
+        S->inflight.clear();
+        return;
+      }
+      if (S->err == WF_OK && S->inflight.size() < 2 && schedulable()) {
+        ci = S->next_ctx;
+        ServeStep& c = S->step[ci];
+        c.finishing.clear();
+        c.nslot = 0;
+        u32 nsched = 0;
+        for (u32 k = 0; k < S->slots; k++)
+          if (S->slot_ticket[k] && !S->reqs[S->slot_ticket[k]].scheduled) nsched++;
+        const u64 share = std::max<u64>(S->min_piece, S->step_numbers / std::max<u32>(nsched, 1));
+        for (u32 k = 0; k < S->slots; k++) {
+          const u64 t = S->slot_ticket[k];
+          if (!t) continue;
+          c.nslot = k + 1;
+          ServeReq& q = S->reqs[t];
+          if (q.scheduled) continue;
+          const bool last = span_d(q.next, q.hi) <= (double)share;
+          const u64 end = last ? q.hi : q.next + share;
+          split_classes(e, q.next, end, k, all);
+          q.next = end;
+          if (last) {
+            q.scheduled = true;
+            c.finishing.push_back(t);
+          }
+        }
+        inits.swap(S->fresh);
       }
     }
-    int rc = WF_OK;
-    u64 maxtop = 0;
-    for (const Piece& p : all) maxtop = std::max<u64>(maxtop, p.hi ? p.hi - 1 : 0xFFFFFFFFFFFFFFFFull);
-    rc = ensure_sieving_primes(e, std::max<u64>(isqrt_u64(maxtop) + 1, 64));
-    TestOut to;
-    to.acc = S->d_acc;
-    to.cks = S->d_cks;
-    to.sumq = S->d_sumq;
-    to.nb = S->nb;
-    to.ev = e->d_ev;
-    to.ev_cap = e->dev_ev_cap;
-    to.nev = e->d_ctr + 1;
-    to.near_T = 0;
-    for (int cls = A_F64; cls < A_NCLS && rc == WF_OK; cls++) {
-      std::vector<Piece> ps;
-      for (const Piece& p : all)
-        if (p.arith == cls) ps.push_back(p);
-      if (!ps.empty()) rc = run_pieces(e, ps, cls, to);
+    int rc;
+    if (ci >= 0) {
+      // a step chosen by the testing knob fails as a HIP error would, before anything is enqueued
+      rc = e->cfg.fault_step && ++S->planned == e->cfg.fault_step ? WF_E_DEVICE : serve_enqueue(e, S, ci, all, inits);
+      if (rc == WF_OK) S->next_ctx ^= 1;
+    } else {
+      rc = serve_finish_oldest(e, S);
     }
-    std::vector<DevAcc> acc(S->slots);
-    std::vector<unsigned long long> cks((size_t)S->slots * S->nb), sumq((size_t)S->slots * S->nb);
-    if (rc == WF_OK &&
-        (hipMemcpy(acc.data(), S->d_acc, S->slots * sizeof(DevAcc), hipMemcpyDeviceToHost) != hipSuccess ||
-         hipMemcpy(cks.data(), S->d_cks, cks.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess ||
-         hipMemcpy(sumq.data(), S->d_sumq, sumq.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) !=
-             hipSuccess))
-      rc = WF_E_DEVICE;
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->steps++;
-    if (rc != WF_OK) {
-      S->err = rc;
-      for (auto& kv : S->reqs) kv.second.done = true;
-      S->cv.notify_all();
-      continue;
-    }
-    for (auto& sc : sched) {
-      ServeReq& q = S->reqs[sc.first];
-      q.next = sc.second;
-      if (q.next != q.hi) continue;
-      // finished: totals of its slot, plus the untested prime 2
-      const DevAcc& a = acc[q.slot];
-      memset(&q.res, 0, sizeof q.res);
-      q.res.primes = a.primes + ((q.lo <= 2 && (q.hi == 0 || q.hi > 2)) ? 1 : 0);
-      q.res.tests = a.tests;
-      q.res.near = a.near;
-      q.res.flt = a.flt;
-      for (u32 j = 0; j < S->nb; j++) {
-        q.res.cks[j] = cks[(size_t)q.slot * S->nb + j];
-        q.res.sumq[j] = sumq[(size_t)q.slot * S->nb + j];
-      }
-      q.res.nev = a.nev;
-      if (q.lo <= 2 && (q.hi == 0 || q.hi > 2))      // the p = 2 solutions, as in wf_scan
-        for (int j = 0; j < e->nb; j++) q.res.nev += e->bases[j] % 4 == 1 ? 1 : 0;
-      q.done = true;
-      (void)hipMemset(S->d_acc + q.slot, 0, sizeof(DevAcc));
-      (void)hipMemset(S->d_cks + (size_t)q.slot * S->nb, 0, S->nb * sizeof(unsigned long long));
-      (void)hipMemset(S->d_sumq + (size_t)q.slot * S->nb, 0, S->nb * sizeof(unsigned long long));
-      S->slot_ticket[q.slot] = 0;
-      S->active--;
-    }
-    S->cv.notify_all();
+    if (rc != WF_OK) serve_fail(e, S, rc);
   }
+}
+
+static void serve_free(Server* S) {
+  (void)hipFree(S->d_acc);
+  (void)hipFree(S->d_cks);
+  (void)hipFree(S->d_sumq);
+  (void)hipFree(S->d_nearv);
+  for (ServeStep& c : S->step) {
+    (void)hipFree(c.d_segs);
+    (void)hipFree(c.d_runs);
+    (void)hipFree(c.d_fills);
+    (void)hipFree(c.d_ev);
+    (void)hipFree(c.d_nev);
+    (void)hipHostFree(c.h_acc);
+    (void)hipHostFree(c.h_cks);
+    (void)hipHostFree(c.h_sumq);
+    (void)hipHostFree(c.h_nev);
+    (void)hipHostFree(c.h_flags);
+    if (c.done) (void)hipEventDestroy(c.done);
+  }
+  delete S;
 }
 
 extern "C" int wf_serve_start(wf_engine* e, u32 slots, u64 hbm_budget) {
@@ -2769,41 +3039,45 @@ extern "C" int wf_serve_start(wf_engine* e, u32 slots, u64 hbm_budget) {
   S->slots = slots;
   S->slot_ticket.assign(slots, 0);
   // bytes of pass buffers per number: prime regions (survivor lists in prove mode, at most
-  // a tenth of the numbers for bucket_min >= 512) + buckets (large primes up to 2^32)
+  // a tenth of the numbers for bucket_min >= 512) + buckets (large primes up to 2^32); two
+  // buffer sets are in use at once
   const double SB = e->cfg.seg_bytes;
   const double pcap = std::max((double)prime_bound(30.0 * SB), e->cfg.prove == WF_PROVE_OFF ? 0.0 : 3.0 * SB);
   const double bcap = 8.0 * SB * 0.70 + 10.0 * std::sqrt(8.0 * SB * 0.70) + 1024.0;
-  const double per_number = (pcap + bcap) * 4.0 / (30.0 * SB);
+  const double per_number = 2.0 * (pcap + bcap) * 4.0 / (30.0 * SB);
   if (!hbm_budget) hbm_budget = 16ull << 30;
   S->step_numbers = (u64)std::min(2.0e10, (double)hbm_budget / per_number);
   S->min_piece = std::max<u64>(30ull * e->cfg.seg_bytes, std::min<u64>(S->step_numbers / 64, 200000000ull));
   if (S->step_numbers < S->min_piece) { delete S; return WF_E_ARG; }
   S->nb = (u32)e->nb;
   const size_t nck = (size_t)slots * S->nb * sizeof(unsigned long long);
-  if (hipMalloc(&S->d_acc, slots * sizeof(DevAcc)) != hipSuccess) { delete S; return WF_E_NOMEM; }
-  if (hipMalloc(&S->d_cks, nck) != hipSuccess || hipMalloc(&S->d_sumq, nck) != hipSuccess ||
-      hipMemset(S->d_cks, 0, nck) != hipSuccess || hipMemset(S->d_sumq, 0, nck) != hipSuccess) {
-    (void)hipFree(S->d_acc);
-    (void)hipFree(S->d_cks);
-    (void)hipFree(S->d_sumq);
-    delete S;
+  bool ok = hipMalloc(&S->d_acc, slots * sizeof(DevAcc)) == hipSuccess && hipMalloc(&S->d_cks, nck) == hipSuccess &&
+            hipMalloc(&S->d_sumq, nck) == hipSuccess && hipMalloc(&S->d_nearv, slots * sizeof(i64)) == hipSuccess;
+  for (ServeStep& c : S->step)
+    ok = ok && hipMalloc(&c.d_ev, SERVE_EVCAP * sizeof(DevEvent)) == hipSuccess &&
+         hipMalloc(&c.d_nev, sizeof(unsigned long long)) == hipSuccess &&
+         hipHostMalloc(&c.h_acc, slots * sizeof(DevAcc), 0) == hipSuccess &&
+         hipHostMalloc(&c.h_cks, nck, 0) == hipSuccess && hipHostMalloc(&c.h_sumq, nck, 0) == hipSuccess &&
+         hipHostMalloc(&c.h_nev, sizeof(unsigned long long), 0) == hipSuccess &&
+         hipHostMalloc(&c.h_flags, sizeof(u32), 0) == hipSuccess &&
+         hipEventCreateWithFlags(&c.done, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    serve_free(S);
     return WF_E_NOMEM;
   }
-  int rc;
-  if ((rc = grow(e->d_ev, e->dev_ev_cap, 1 << 16)) != WF_OK || hipMemset(S->d_acc, 0, slots * sizeof(DevAcc)) != hipSuccess ||
-      hipMemset(e->d_flags, 0, sizeof(u32)) != hipSuccess || hipMemset(e->d_ctr, 0, 3 * sizeof(unsigned long long)) != hipSuccess) {
-    (void)hipFree(S->d_acc);
-    (void)hipFree(S->d_cks);
-    (void)hipFree(S->d_sumq);
-    delete S;
-    return rc != WF_OK ? rc : WF_E_DEVICE;
+  // set up once, before any step: every slot starts cleared
+  if (hipMemset(S->d_acc, 0, slots * sizeof(DevAcc)) != hipSuccess || hipMemset(S->d_cks, 0, nck) != hipSuccess ||
+      hipMemset(S->d_sumq, 0, nck) != hipSuccess || hipMemset(S->d_nearv, 0, slots * sizeof(i64)) != hipSuccess ||
+      hipMemset(e->d_flags, 0, sizeof(u32)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    serve_free(S);
+    return WF_E_DEVICE;
   }
   e->srv = S;
   S->th = std::thread(serve_loop, e);
   return WF_OK;
 }
 
-extern "C" int wf_serve_submit(wf_engine* e, u64 lo, u64 hi, u64* ticket) {
+extern "C" int wf_serve_submit(wf_engine* e, u64 lo, u64 hi, int64_t near_T, u64* ticket) {
   if (!e || !e->srv || !ticket || (hi != 0 && hi <= lo)) return WF_E_ARG;
   Server* S = e->srv;
   std::lock_guard<std::mutex> lk(S->mu);
@@ -2813,34 +3087,38 @@ extern "C" int wf_serve_submit(wf_engine* e, u64 lo, u64 hi, u64* ticket) {
   u32 k = 0;
   while (S->slot_ticket[k]) k++;
   const u64 t = S->next_ticket++;
-  ServeReq q;
+  ServeReq& q = S->reqs[t];
   q.lo = lo;
   q.hi = hi;
   q.next = lo;
+  q.near_T = near_T;
   q.slot = k;
+  q.scheduled = false;
   q.done = false;
+  q.rc = WF_OK;
   memset(&q.res, 0, sizeof q.res);
-  S->reqs[t] = q;
   S->slot_ticket[k] = t;
+  S->fresh.push_back({k, near_T});
   S->active++;
   *ticket = t;
   S->cv.notify_all();
   return WF_OK;
 }
 
-extern "C" int wf_serve_poll(wf_engine* e, u64 ticket, wf_result* out, int* done) {
-  if (!e || !e->srv || !out || !done) return WF_E_ARG;
+extern "C" int wf_serve_poll(wf_engine* e, u64 ticket, wf_result* out, wf_event* ev, u64 ev_cap, int* done) {
+  if (!e || !e->srv || !out || !done || (ev_cap && !ev)) return WF_E_ARG;
   Server* S = e->srv;
   std::lock_guard<std::mutex> lk(S->mu);
   auto it = S->reqs.find(ticket);
   if (it == S->reqs.end()) return WF_E_ARG;
-  if (S->err) return S->err;
-  *done = it->second.done ? 1 : 0;
-  if (*done) {
-    *out = it->second.res;
-    S->reqs.erase(it);
-  }
-  return WF_OK;
+  ServeReq& q = it->second;
+  *done = q.done ? 1 : 0;
+  if (!q.done) return S->err;
+  *out = q.res;
+  for (u64 k = 0; k < q.ev.size() && k < ev_cap; k++) to_event(q.ev[k], 0, ev[k]);
+  const int rc = q.rc;
+  S->reqs.erase(it);
+  return rc;
 }
 
 extern "C" int wf_serve_stop(wf_engine* e) {
@@ -2852,11 +3130,8 @@ extern "C" int wf_serve_stop(wf_engine* e) {
   }
   S->cv.notify_all();
   if (S->th.joinable()) S->th.join();
-  (void)hipFree(S->d_acc);
-  (void)hipFree(S->d_cks);
-  (void)hipFree(S->d_sumq);
   const int err = S->err;
-  delete S;
+  serve_free(S);
   e->srv = nullptr;
   return err;
 }

@@ -19,7 +19,8 @@ u64, u32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64
 
 
 class Event(ctypes.Structure):
-    _fields_ = [("type", u32), ("j", u32), ("p", u64), ("q", u64), ("r_hi", u64), ("r_lo", u64)]
+    _fields_ = [("type", u32), ("j", u32), ("p", u64), ("q", u64), ("r_hi", u64), ("r_lo", u64), ("range", u32),
+                ("pad", u32)]
 
 
 class Result(ctypes.Structure):
@@ -41,7 +42,7 @@ class Range(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("seg_bytes", u32), ("sieve_block", u32), ("test_block", u32), ("arith", u32), ("bucket_min", u32),
-                ("chunk", u64), ("test_wgs", u32), ("prove", u32), ("derive", u32)]
+                ("chunk", u64), ("test_wgs", u32), ("prove", u32), ("derive", u32), ("fault_step", u32), ("fault_p", u64)]
 
 
 class Timing(ctypes.Structure):
@@ -88,8 +89,9 @@ def load():
     L.wf_autotune_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Range), u32, ctypes.POINTER(Config),
                                     ctypes.POINTER(ctypes.c_double)]
     L.wf_serve_start.argtypes = [ctypes.c_void_p, u32, u64]
-    L.wf_serve_submit.argtypes = [ctypes.c_void_p, u64, u64, P64]
-    L.wf_serve_poll.argtypes = [ctypes.c_void_p, u64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_int)]
+    L.wf_serve_submit.argtypes = [ctypes.c_void_p, u64, u64, i64, P64]
+    L.wf_serve_poll.argtypes = [ctypes.c_void_p, u64, ctypes.POINTER(Result), ctypes.POINTER(Event), u64,
+                                ctypes.POINTER(ctypes.c_int)]
     L.wf_serve_stop.argtypes = [ctypes.c_void_p]
     for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_quotients_base",
               "wf_primes", "wf_bpsw", "wf_sieve_stats",
@@ -124,9 +126,12 @@ class Engine:
     """One GPU engine (wf_open); not thread-safe, like the C object."""
 
     def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0, bucket_min=0, test_wgs=0,
-                 prove=PROVE_AUTO, derive=DERIVE_AUTO):
+                 prove=PROVE_AUTO, derive=DERIVE_AUTO, fault_p=0, fault_step=0):
+        """fault_p and fault_step are for testing only: the residue of p = fault_p is taken as
+        r + 1, so its pair fails the FLT check and is reported as an FLT event; serving step
+        number fault_step fails (every request then ends with WF_E_DEVICE)."""
         L = load()
-        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs, prove, derive)
+        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs, prove, derive, fault_step, fault_p)
         h = ctypes.c_void_p()
         _check(L.wf_open(device, ctypes.byref(cfg), ctypes.byref(h)), "wf_open")
         self._h = h
@@ -161,7 +166,9 @@ class Engine:
         _check(self._L.wf_set_bases(self._h, arr, len(bases)), "wf_set_bases")
         self._bases = list(bases)
 
-    def _events(self, ev, n):
+    def _events(self, ev, n, with_range=False):
+        if with_range:
+            return [(e.type, e.p, e.q, (e.r_hi << 64) | e.r_lo, self._bases[e.j], e.range) for e in ev[:n]]
         return [(e.type, e.p, e.q, (e.r_hi << 64) | e.r_lo, self._bases[e.j]) for e in ev[:n]]
 
     def scan(self, lo, hi, near_T=0, ev_cap=1 << 16, psp=False, psp_cap=1 << 16):
@@ -176,13 +183,15 @@ class Engine:
         return out
 
     def scan_batch(self, ranges, near_T=0, ev_cap=1 << 16):
+        """Per-range totals and the events of all ranges as (type, p, q, r, base, range index),
+        sorted by (p, base, range)."""
         n = len(ranges)
         rr = (Range * n)(*[Range(lo, _hi(hi)) for lo, hi in ranges])
         res = (Result * n)()
         ev = (Event * max(ev_cap, 1))()
         _check(self._L.wf_scan_batch(self._h, rr, n, near_T, res, ev, ev_cap), "wf_scan_batch")
         total_ev = sum(r.nev for r in res)
-        events = self._events(ev, min(total_ev, ev_cap))
+        events = self._events(ev, min(total_ev, ev_cap), with_range=True)
         return [result_dict(r, None, len(self._bases)) for r in res], events
 
     def quotients(self, ps, base=2):
@@ -238,20 +247,25 @@ class Engine:
     def serve_start(self, slots=64, hbm_budget=0):
         _check(self._L.wf_serve_start(self._h, slots, hbm_budget), "wf_serve_start")
 
-    def serve_submit(self, lo, hi):
+    def serve_submit(self, lo, hi, near_T=0):
         """Ticket, or None when the server is full (WF_E_BUSY: retry later)."""
         t = u64()
-        rc = self._L.wf_serve_submit(self._h, lo, _hi(hi), ctypes.byref(t))
+        rc = self._L.wf_serve_submit(self._h, lo, _hi(hi), near_T, ctypes.byref(t))
         if rc == -6:
             return None
         _check(rc, "wf_serve_submit")
         return t.value
 
-    def serve_poll(self, ticket):
-        """Result dict once the request is done, else None."""
+    def serve_poll(self, ticket, ev_cap=1 << 16):
+        """Result dict (with its events, as scan returns them) once the request is done, else
+        None.  Raises WFError (WF_E_OVERFLOW) for a request whose events overflowed."""
         res, done = Result(), ctypes.c_int()
-        _check(self._L.wf_serve_poll(self._h, ticket, ctypes.byref(res), ctypes.byref(done)), "wf_serve_poll")
-        return result_dict(res, None, len(self._bases)) if done.value else None
+        ev = (Event * max(ev_cap, 1))()
+        _check(self._L.wf_serve_poll(self._h, ticket, ctypes.byref(res), ev, ev_cap, ctypes.byref(done)),
+               "wf_serve_poll")
+        if not done.value:
+            return None
+        return result_dict(res, self._events(ev, min(res.nev, ev_cap)), len(self._bases))
 
     def serve_stop(self):
         _check(self._L.wf_serve_stop(self._h), "wf_serve_stop")

@@ -434,7 +434,9 @@ void wfo_redc(u64 p, u64 x_hi, u64 x_lo, u64 *r_hi, u64 *r_lo)
 
 static u32 *g_sp = NULL;        /* odd primes 3 .. g_sp_lim */
 static u64 g_nsp = 0, g_sp_lim = 0;
-static pthread_mutex_t g_sp_mu = PTHREAD_MUTEX_INITIALIZER;
+/* Scans read the table under the read lock; growing it takes the write lock, so scans may
+ * run concurrently (the tests run many at once from threads). */
+static pthread_rwlock_t g_sp_rw = PTHREAD_RWLOCK_INITIALIZER;
 
 static u64 isqrt64(u64 n)
 {
@@ -444,11 +446,19 @@ static u64 isqrt64(u64 n)
     return r;
 }
 
-/* Make g_sp hold every odd prime <= lim (lim <= 2^32). */
+/* Make g_sp hold every odd prime <= lim (lim <= 2^32) and return holding the table's read
+ * lock: the caller reads g_sp, then calls sp_release.  The table only grows. */
 static void sieving_primes(u64 lim)
 {
-    pthread_mutex_lock(&g_sp_mu);
-    if (lim <= g_sp_lim) { pthread_mutex_unlock(&g_sp_mu); return; }
+    pthread_rwlock_rdlock(&g_sp_rw);
+    if (lim <= g_sp_lim) return;
+    pthread_rwlock_unlock(&g_sp_rw);
+    pthread_rwlock_wrlock(&g_sp_rw);
+    if (lim <= g_sp_lim) {                      /* another thread grew it meanwhile */
+        pthread_rwlock_unlock(&g_sp_rw);
+        pthread_rwlock_rdlock(&g_sp_rw);
+        return;
+    }
     if (lim < 1024) lim = 1024;
     /* bit i of the odd-only table = 2i + 1 */
     const u64 nbits = lim / 2 + 1;
@@ -468,8 +478,11 @@ static void sieving_primes(u64 lim)
     g_sp = sp;
     g_nsp = n;
     g_sp_lim = lim;
-    pthread_mutex_unlock(&g_sp_mu);
+    pthread_rwlock_unlock(&g_sp_rw);
+    pthread_rwlock_rdlock(&g_sp_rw);
 }
+
+static void sp_release(void) { pthread_rwlock_unlock(&g_sp_rw); }
 
 #define SEG_ODDS (1u << 21)     /* odd numbers per sieve segment (256 KiB bitmap) */
 
@@ -554,14 +567,16 @@ uint64_t wfo_primes(u64 lo, u64 hi, u64 *out, u64 cap, int method)
     u64 n = 0;
     if (lo <= 2 && (hi == 0 || hi > 2)) { if (n < cap) out[n] = 2; n++; }
     oddsieve_t S;
-    if (!oddsieve_init(&S, lo, hi)) return n;
-    while (S.seg < S.nodd) {
-        const u64 len = oddsieve_segment(&S);
-        for (u64 i = 0; i < len; i++)
-            if (!(S.bits[i >> 6] >> (i & 63) & 1)) { if (n < cap) out[n] = S.s0 + 2 * (S.seg + i); n++; }
-        S.seg += len;
+    if (oddsieve_init(&S, lo, hi)) {
+        while (S.seg < S.nodd) {
+            const u64 len = oddsieve_segment(&S);
+            for (u64 i = 0; i < len; i++)
+                if (!(S.bits[i >> 6] >> (i & 63) & 1)) { if (n < cap) out[n] = S.s0 + 2 * (S.seg + i); n++; }
+            S.seg += len;
+        }
+        oddsieve_free(&S);
     }
-    oddsieve_free(&S);
+    sp_release();
     return n;
 }
 
@@ -750,6 +765,7 @@ int wfo_scan(u64 lo, u64 hi, const u32 *bases, int nb, int64_t near_T, int threa
         if (pthread_create(&th[t], NULL, worker_main, &W[t]) != 0) worker_main(&W[t]), th[t] = 0;
     }
     for (int t = 0; t < threads; t++) if (th[t]) pthread_join(th[t], NULL);
+    sp_release();
     memset(res, 0, sizeof *res);
     u64 nev = 0, npsp = 0;
     for (int t = 0; t < threads; t++) {
@@ -763,9 +779,9 @@ int wfo_scan(u64 lo, u64 hi, const u32 *bases, int nb, int64_t near_T, int threa
     u64 *allp = malloc((npsp ? npsp : 1) * sizeof *allp);
     u64 k = 0, kp = 0;
     for (int t = 0; t < threads; t++) {
-        memcpy(all + k, W[t].acc.ev, W[t].acc.nev * sizeof *all);
+        if (W[t].acc.nev) memcpy(all + k, W[t].acc.ev, W[t].acc.nev * sizeof *all);   /* no copy from NULL */
         k += W[t].acc.nev;
-        memcpy(allp + kp, W[t].acc.psp, W[t].acc.npsp * sizeof *allp);
+        if (W[t].acc.npsp) memcpy(allp + kp, W[t].acc.psp, W[t].acc.npsp * sizeof *allp);
         kp += W[t].acc.npsp;
         free(W[t].acc.ev);
         free(W[t].acc.psp);

@@ -273,3 +273,95 @@ def test_dcheck_other_bases_and_top_of_word(tmp_path):
         rc, out, err = run("dcheck", log, bases, "1", "1", outp)
         assert rc == 0, err
         assert [r.split()[0] for r in outp.read_text().splitlines()] == ["MATCH", "MATCH"]
+
+
+# ---- every output line pinned, the FLT-error path, event overflow ----
+
+M64 = (1 << 64) - 1
+ANUM = {2: "A001220", 3: "A014127", 5: "A123692"}
+
+
+def expected_stdout(start, end, chunk, bases, near_T, oracle):
+    """The scan's whole stdout built from the oracle's results in the reference's formats:
+    per chunk its FOUND and NEAR lines in (p, base) order (wieferich.c:1009, :1071), then a
+    CONFIRMED line per solution of the chunk (:1350-1352); then SUMMARY (:1505-1523)."""
+    out, hits, tot = [], [], {"primes": 0, "tests": 0, "near": 0, "cks": {b: 0 for b in bases}}
+    for lo in range(start, end, chunk):
+        hi = min(lo + chunk, end)
+        o = oracle.scan(lo, hi, bases=bases, near_T=near_T)
+        found = []
+        for t, b, p, q, r in o["events"]:
+            if t == 0:
+                out.append("FOUND base %d p %d (%s)" % (b, p, ANUM[b]))
+                found.append((b, p))
+            elif t == 1:
+                out.append("NEAR base %d p %d A %d" % (b, p, q if q <= p // 2 else q - p))
+        for b, p in found:
+            out.append("CONFIRMED base %d p %d (%s) (all primes in [%d, %d) tested)" % (b, p, ANUM[b], start, hi))
+        hits += found
+        for k in ("primes", "tests", "near"):
+            tot[k] += o[k]
+        for b in bases:
+            tot["cks"][b] = (tot["cks"][b] + o["cks"][b]) & M64
+    out.append("SUMMARY [%d, %d) primes %d tests %d flt_errors 0" % (start, end, tot["primes"], tot["tests"]) +
+               (" near(|A|<=%d) %d" % (near_T, tot["near"]) if near_T > 0 else ""))
+    for b in bases:
+        sol = [str(p) for bb, p in hits if bb == b]
+        out.append("base %3d %-8s cks %016x  solutions: %s" % (b, ANUM[b], tot["cks"][b], " ".join(sol) or "none"))
+    return "\n".join(out) + "\n", tot, hits
+
+
+@pytest.mark.gpu
+def test_whole_output_and_checkpoint_text(tmp_path):
+    """wf scan's complete stdout and the complete checkpoint it leaves, byte for byte."""
+    import oracle
+    ck = tmp_path / "st"
+    rc, out, err = run("scan", "0", "2e5", "-c", "5e4", "-b", "2,3,5", "-n", "3", "-q", "-S", ck)
+    assert rc == 0, err
+    want, tot, hits = expected_stdout(0, 200000, 50000, [2, 3, 5], 3, oracle)
+    assert out == want
+    text = ("wieferich checkpoint 1\nbases 2,3,5\nstart 0\nchunk 50000\nend 200000\nfrontier 4\nprimes %d\n"
+            "tests %d\nnear %d\nflt 0\n" % (tot["primes"], tot["tests"], tot["near"]) +
+            "".join("cks %d %016x\n" % (b, tot["cks"][b]) for b in (2, 3, 5)))
+    text += "".join("hit %d %d %d\n" % (b, p, p // 50000) for b, p in hits) + "end\n"
+    assert ck.read_text() == text
+    # resuming a complete scan re-announces its solutions and prints the same SUMMARY
+    rc, out2, err = run("scan", "0", "2e5", "-c", "5e4", "-b", "2,3,5", "-n", "3", "-q", "-S", ck)
+    assert rc == 0 and out2.splitlines()[-4:] == out.splitlines()[-4:]
+    assert [l for l in out2.splitlines() if l.startswith("CONFIRMED")] == \
+        ["CONFIRMED base %d p %d (%s) (all primes in [0, 200000) tested)" % (b, p, ANUM[b]) for b, p in hits]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lo", [10 ** 15, 10 ** 18])
+def test_flt_error_line_and_exit_status(tmp_path, lo):
+    """--fault-p P corrupts P's residue on the GPU (r -> r + 1): the scan prints the reference's
+    FLT-ERROR line with that residue, counts the pair in tests and flt_errors, leaves it out of
+    the checksum, the sum of q and the chunk log, and exits with status 3
+    (wieferich.c:1058-1063, :1591)."""
+    import oracle
+    hi = lo + 2 * 10 ** 6
+    P = oracle.primes(lo + 1500000, lo + 1510000)[2]
+    r = pow(2, P - 1, P * P)
+    q = r // P
+    log = tmp_path / "log"
+    rc, out, err = run("scan", lo, hi, "-c", "1e6", "-b", "2", "-q", "-Q", "-L", log, "--fault-p", P)
+    assert rc == 3, err
+    assert "FLT-ERROR base 2 p %d r = %032x" % (P, r + 1) in out.splitlines()
+    o = oracle.scan(lo, hi, bases=[2])
+    cks = (o["cks"][2] - oracle.mix64(q ^ (P * 0x9E3779B97F4A7C15 & M64))) & M64
+    assert "SUMMARY [%d, %d) primes %d tests %d flt_errors 1" % (lo, hi, o["primes"], o["tests"]) in out
+    assert re.search(r"^base   2 A001220  cks %016x  solutions: none$" % cks, out, re.M)
+    assert "sumq base 2 %016x" % ((o["sumq"][2] - q) & M64) in out
+    term = oracle.mix64(q ^ (P * 0x9E3779B97F4A7C15 & M64))      # P's checksum term, left out
+    first, second = log.read_text().splitlines()
+    o2 = oracle.scan(lo + 10 ** 6, hi, bases=[2])
+    assert second == "C %d %d %d 2:%016x" % (lo + 10 ** 6, hi, o2["primes"], (o2["cks"][2] - term) & M64)
+
+
+@pytest.mark.gpu
+def test_event_overflow_message():
+    """More events in one chunk than the host's buffer of 65536 (every prime a NEAR event
+    under -n 1e18) ends the scan with a message and status 1, never a silent truncation."""
+    rc, out, err = run("scan", "1e9", "1e9+2e6", "-c", "2e6", "-b", "2", "-n", "1e18", "-q")
+    assert rc == 1 and "events exceed the event buffer (65536); use a smaller -n or CHUNK" in err, err
