@@ -56,6 +56,7 @@ enum {
     WF_E_BUSY = -6,         /* serving: no free slot / HBM budget; or a call that needs the
                                engine (scans, quotients, BPSW, statistics, autotuning,
                                timings, bases) while it serves, until wf_serve_stop */
+    WF_E_LIMIT = -7,        /* wf_quotients_static: n above WF_STATIC_QUOT_MAX */
 };
 
 /* Event types, the reference's result lines (wieferich.c:1009, :1060, :1071). */
@@ -153,6 +154,16 @@ int  wf_quotients(wf_engine *e, const uint64_t *p, uint64_t *q, uint64_t *r_hi,
  * when p | b). */
 int  wf_quotients_base(wf_engine *e, uint32_t b, const uint64_t *p, uint64_t *q,
                        uint64_t *r_hi, uint64_t *r_lo, uint64_t n);
+
+/* wf_quotients_base for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  It calls no allocating routine: its staging area is a static device array of
+ * WF_STATIC_QUOT_MAX values per GPU (8 MiB, held until the process ends), so n above that
+ * returns WF_E_LIMIT.  Every call on one GPU shares the array: calls that overlap on one GPU
+ * race. */
+#define WF_STATIC_QUOT_MAX (1u << 18)
+int  wf_quotients_static(wf_engine *e, uint32_t b, const uint64_t *p, uint64_t *q,
+                         uint64_t *r_hi, uint64_t *r_lo, uint64_t n);
 
 /* The primes of [lo, hi) in ascending order into out[0..cap); *count = how many exist. */
 int  wf_primes(wf_engine *e, uint64_t lo, uint64_t hi, uint64_t *out, uint64_t cap,

@@ -1272,6 +1272,30 @@ __global__ void k_zero(u32* __restrict__ p, u64 n) {
 
 static const u32 R30H[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 
+// Staging area of wf_quotients_static, the entry point that allocates nothing: p, q, r_hi
+// and r_lo of up to WF_STATIC_QUOT_MAX values (8 MiB).  Each GPU the library runs on holds
+// its own copy until the process ends.  In host code the name is only a symbol: the device
+// address of a GPU's copy comes from hipGetSymbolAddress with that GPU current, taken once
+// per device by wf_open (scratch_addr) and used wherever host code hands the array out.
+static __device__ __align__(256) u64 scratch[4 * (size_t)WF_STATIC_QUOT_MAX];
+
+#define WF_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static u64* g_scratch_ptr[WF_MAX_DEVICES];
+
+// The device address of `device`'s scratch array (the device must be current).
+static int scratch_addr(int device, u64** out) {
+  if (device < 0 || device >= WF_MAX_DEVICES) return WF_E_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return WF_E_DEVICE;
+    g_scratch_ptr[device] = (u64*)ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return WF_OK;
+}
+
 struct BufSet {   // bucket counters / records, prime offsets / counts of one pass
   u32* bcnt = nullptr;
   u64 bcnt_cap = 0;
@@ -1338,6 +1362,7 @@ struct wf_engine {
   hipEvent_t ev[4] = {};
   wf_timing last;
   struct Server* srv = nullptr;
+  u64* scratch = nullptr;          // this device's static scratch array (wf_quotients_static)
 };
 
 #define HIPCHK(x)                                  \
@@ -1354,6 +1379,7 @@ extern "C" const char* wf_strerror(int rc) {
     case WF_E_UNSUPPORTED: return "not supported by the GPU engine (base 2 only)";
     case WF_E_OVERFLOW: return "internal buffer bound exceeded";
     case WF_E_BUSY: return "engine busy serving (no free slot, or wf_serve_stop not called yet)";
+    case WF_E_LIMIT: return "more values than the fixed scratch area of wf_quotients_static holds";
     default: return "unknown error";
   }
 }
@@ -1494,6 +1520,10 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
     return WF_E_ARG;
   }
   e->sbshift = __builtin_ctz(e->cfg.seg_bytes);
+  if (scratch_addr(device, &e->scratch) != WF_OK) {
+    delete e;
+    return WF_E_DEVICE;
+  }
   // segments above 64 KiB need the kernel's dynamic LDS limit raised
   if (hipFuncSetAttribute((const void*)k_sieve, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)std::min<size_t>(e->prop.sharedMemPerBlock - 4096, SEG_MAX)) != hipSuccess) {
@@ -2400,32 +2430,52 @@ extern "C" int wf_scan_batch(wf_engine* e, const wf_range* r, u32 n, int64_t nea
   return scan_ranges(e, r, n, near_T, per_range, ev, ev_cap, true);
 }
 
-extern "C" int wf_quotients_base(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
+static int quotient_args(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
   if (!e || !p || !q || !r_hi || !r_lo || b < 2) return WF_E_ARG;
   if (e->srv) return WF_E_BUSY;
-  if (n == 0) return WF_OK;
   for (u64 i = 0; i < n; i++)
     if (p[i] < 3 || !(p[i] & 1)) return WF_E_ARG;
   if (e->cfg.arith == 1 && b != 2) return WF_E_UNSUPPORTED;
-  HIPCHK(hipSetDevice(e->dev));
+  return WF_OK;
+}
+
+// k_quot over n values staged in d[0 .. 4n): p in, then q, r_hi, r_lo out.
+static int quotients_in(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n, u64* d) {
   int rc;
   if ((rc = put_base(WF_MAXB, b)) != WF_OK) return rc;
+  HIPCHK(hipMemcpyAsync(d, p, n * sizeof(u64), hipMemcpyHostToDevice, e->st));
+  hipLaunchKernelGGL(k_quot, dim3((u32)((n + 255) / 256)), dim3(256), 0, e->st, d, d + n, d + 2 * n, d + 3 * n, n,
+                     e->cfg.arith == 1 ? 1 : 0, (u32)WF_MAXB);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(q, d + n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipMemcpyAsync(r_hi, d + 2 * n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipMemcpyAsync(r_lo, d + 3 * n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  return WF_OK;
+}
+
+extern "C" int wf_quotients_base(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
+  int rc;
+  if ((rc = quotient_args(e, b, p, q, r_hi, r_lo, n)) != WF_OK || n == 0) return rc;
+  HIPCHK(hipSetDevice(e->dev));
   u64* d = nullptr;
   if (hipMalloc(&d, 4 * n * sizeof(u64)) != hipSuccess) return WF_E_NOMEM;
-  rc = WF_OK;
-  if (hipMemcpyAsync(d, p, n * sizeof(u64), hipMemcpyHostToDevice, e->st) != hipSuccess) rc = WF_E_DEVICE;
-  if (rc == WF_OK) {
-    hipLaunchKernelGGL(k_quot, dim3((u32)((n + 255) / 256)), dim3(256), 0, e->st, d, d + n, d + 2 * n, d + 3 * n, n,
-                       e->cfg.arith == 1 ? 1 : 0, (u32)WF_MAXB);
-    if (hipGetLastError() != hipSuccess) rc = WF_E_DEVICE;
-  }
-  if (rc == WF_OK && (hipMemcpyAsync(q, d + n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st) != hipSuccess ||
-                      hipMemcpyAsync(r_hi, d + 2 * n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st) != hipSuccess ||
-                      hipMemcpyAsync(r_lo, d + 3 * n, n * sizeof(u64), hipMemcpyDeviceToHost, e->st) != hipSuccess ||
-                      hipStreamSynchronize(e->st) != hipSuccess))
-    rc = WF_E_DEVICE;
+  rc = quotients_in(e, b, p, q, r_hi, r_lo, n, d);
+  (void)hipStreamSynchronize(e->st);    // an early error return leaves nothing queued on d
   (void)hipFree(d);
   return rc;
+}
+
+// The no-allocation variant: the staging area is the device's static scratch array, whose
+// address wf_open took once per device (scratch_addr).  Every call on a GPU shares it, so
+// calls that overlap on one GPU race; the area holds the last call's values afterwards.
+extern "C" int wf_quotients_static(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
+  int rc;
+  if ((rc = quotient_args(e, b, p, q, r_hi, r_lo, n)) != WF_OK || n == 0) return rc;
+  if (n > WF_STATIC_QUOT_MAX) return WF_E_LIMIT;
+  if (!e->scratch) return WF_E_DEVICE;
+  HIPCHK(hipSetDevice(e->dev));
+  return quotients_in(e, b, p, q, r_hi, r_lo, n, e->scratch);
 }
 
 extern "C" int wf_quotients(wf_engine* e, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {

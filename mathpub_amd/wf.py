@@ -14,6 +14,7 @@ EV_FOUND, EV_NEAR, EV_FLT = 0, 1, 2
 PROVE_AUTO, PROVE_OFF, PROVE_ON = 0, 1, 2
 DERIVE_AUTO, DERIVE_OFF = 0, 1     # composite bases from their prime factors' quotients
 BPSW_SPRP2, BPSW_LUCAS, BPSW_PRIME = 1, 2, 4
+STATIC_QUOT_MAX = 1 << 18          # wf_quotients_static's fixed scratch area (values per call)
 
 u64, u32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64
 
@@ -81,6 +82,7 @@ def load():
                                 ctypes.POINTER(Event), u64]
     L.wf_quotients.argtypes = [ctypes.c_void_p, P64, P64, P64, P64, u64]
     L.wf_quotients_base.argtypes = [ctypes.c_void_p, u32, P64, P64, P64, P64, u64]
+    L.wf_quotients_static.argtypes = [ctypes.c_void_p, u32, P64, P64, P64, P64, u64]
     L.wf_primes.argtypes = [ctypes.c_void_p, u64, u64, P64, u64, P64]
     L.wf_bpsw.argtypes = [ctypes.c_void_p, P64, ctypes.POINTER(ctypes.c_uint8), u64]
     L.wf_sieve_stats.argtypes = [ctypes.c_void_p, u64, u64, ctypes.POINTER(SieveStats)]
@@ -94,6 +96,7 @@ def load():
                                 ctypes.POINTER(ctypes.c_int)]
     L.wf_serve_stop.argtypes = [ctypes.c_void_p]
     for f in ("wf_open", "wf_info", "wf_set_bases", "wf_scan", "wf_scan_batch", "wf_quotients", "wf_quotients_base",
+              "wf_quotients_static",
               "wf_primes", "wf_bpsw", "wf_sieve_stats",
               "wf_last_timing", "wf_autotune", "wf_autotune_batch", "wf_serve_start", "wf_serve_submit", "wf_serve_poll", "wf_serve_stop"):
         getattr(L, f).restype = ctypes.c_int
@@ -200,6 +203,14 @@ class Engine:
         arr = (u64 * n)(*ps)
         q, rh, rl = (u64 * n)(), (u64 * n)(), (u64 * n)()
         _check(self._L.wf_quotients_base(self._h, base, arr, q, rh, rl, n), "wf_quotients_base")
+        return [(q[i], (rh[i] << 64) | rl[i]) for i in range(n)]
+
+    def quotients_static(self, ps, base=2):
+        """The same through wf_quotients_static (no device allocation; at most STATIC_QUOT_MAX)."""
+        n = len(ps)
+        arr = (u64 * max(n, 1))(*ps)
+        q, rh, rl = (u64 * max(n, 1))(), (u64 * max(n, 1))(), (u64 * max(n, 1))()
+        _check(self._L.wf_quotients_static(self._h, base, arr, q, rh, rl, n), "wf_quotients_static")
         return [(q[i], (rh[i] << 64) | rl[i]) for i in range(n)]
 
     def primes(self, lo, hi):
