@@ -24,7 +24,9 @@ NEAR_T = 100
 METRIC = "primes Wieferich-tested/sec at p~1e15, batched vs plain GPU, bit-exact cks"
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector peak (published), FMA = 2 flops
 FP64_PEAK_INSTR = 256 * 64 * 2.4e9   # wave64 FP64 VALU instructions: lanes per second
-INT32_MUL_PEAK = 256 * 32 * 2.4e9    # v_mad_u64_u32 products/s (half rate, measured)
+# 32x32->64 products per second of v_mad_u64_u32 over the whole chip, measured by
+# mathpub_amd/wf_probe valu (profiles/valu_rates.txt: 35.47 lane-ops per clock and CU)
+INT32_MUL_PEAK = 2.179e13
 K_TEST_VALU_PER_TEST = 787           # k_test<A_F64> at [1e15, 1e15+1e11): profiles/c2_sq_counters_pmc.md
 
 
@@ -106,6 +108,43 @@ def _engine_line(metric, value, unit, ms, args, config, extra=None):
     return 0
 
 
+def int_roofline(kernel, products, ms, launches, note):
+    """Roofline of an integer-multiply-bound kernel: algorithmic 32x32 products (the W(p)
+    convention) per second of its HIP-event time, against the measured v_mad_u64_u32 rate.
+    "mfma" is the compute side of the two-way split: no MFMA runs (one modulus per lane)."""
+    achieved = products / (ms / 1e3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": INT32_MUL_PEAK / 1e12, "unit": "Tmul32/s",
+            "frac": achieved / (INT32_MUL_PEAK / 1e12), "traffic": None, "kernel": kernel,
+            "compute_pipe": "integer VALU (v_mad_u64_u32, measured peak); no MFMA: one modulus per lane",
+            "avg_launch_ms": ms / max(launches, 1), "launches": launches, "work": note}
+
+
+def cpu_sample(pieces, target_s):
+    """The oracle (wieferich.c's algorithm) on all usable host cores over the pieces
+    [(lo, hi, kwargs)] in order until it has run target_s seconds: primes tested per second
+    and what was covered."""
+    import oracle
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    tests = done = numbers = 0
+    el = 0.0
+    for lo, hi, kw in pieces:
+        t = time.perf_counter()
+        r = oracle.scan(lo, hi, threads=cores, **kw)
+        el += time.perf_counter() - t
+        tests += r["tests"]
+        numbers += hi - lo
+        done += 1
+        if el >= target_s:
+            break
+    return {"value": tests / el, "unit": "primes tested/s", "cores": cores, "kind": "port",
+            "sample": "%d of the workload's ranges in order (%.3g numbers, %d primes tested) in %.1f s "
+                      "(oracle/wf_oracle.c)" % (done, numbers, tests, el)}
+
+
 def _prove(args):
     from mathpub_amd import wf
     return {"auto": wf.PROVE_AUTO, "off": wf.PROVE_OFF, "on": wf.PROVE_ON}[args.prove]
@@ -144,6 +183,32 @@ def bench_c3(args):
             g = json.load(f)
         golden = ([r["primes"] for r in per] == g["per_range_primes"]
                   and ["%016x" % r["cks"] for r in per] == g["per_range_cks2"])
+    # the plain engine on the same ranges (passes one after another, every kernel timed with
+    # HIP events on its stream): the batched/plain ratio and the test kernel's roofline
+    plain = None
+    roof = None
+    if not args.no_plain:
+        from mathpub_amd import wf as _wf
+        cfg = tuned or {}
+        peng = _wf.Engine(arith=args.arith, prove=_prove(args), pipeline=_wf.PIPELINE_OFF,
+                          seg_bytes=cfg.get("seg_bytes", args.seg_bytes), sieve_block=cfg.get("sieve_block", args.sieve_block),
+                          bucket_min=cfg.get("bucket_min", args.bucket_min))
+        t = time.perf_counter()
+        pper, _ = peng.scan_batch(ranges)
+        pms = (time.perf_counter() - t) * 1e3
+        pt = peng.last_timing()
+        peng.close()
+        assert [(x["primes"], x["cks"]) for x in pper] == [(x["primes"], x["cks"]) for x in per]
+        plain = {"ms_per_step": pms, "primes_per_s": tests / (pms / 1e3), "batched_over_plain": pms / ms,
+                 "sieve_ms": pt["sieve_ms"], "test_ms": pt["test_ms"], "launches": pt["launches"]}
+        roof = int_roofline("k_test<A_M28> (and the one k_test<A_F64> pass of the ranges below 2^50)", prods,
+                            pt["test_ms"], pt["launches"],
+                            "sum over ranges of tests x W(bit length of the range's middle), W = the survey's 32x32 "
+                            "product count per test (1730 at 58 bits); HIP-event time of the plain engine's test launches")
+        print("c3 plain engine: %s" % plain, file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_sample([(lo, hi, {"bases": [2]}) for lo, hi in ranges], 15.0)
     compare = None
     if args.c3_compare:
         # the same first N ranges through the plain single-range path (one wf_scan each)
@@ -169,7 +234,9 @@ def bench_c3(args):
                                    "plain_vs_batched_same_ranges": compare,
                                    "device_ms_per_step": sum(t["total_ms"] for t in tims) / len(tims),
                                    "host_plan_ms_per_step": sum(t["plan_ms"] for t in tims) / len(tims),
-                                   "prove": args.prove, "primes_proved_per_step": tims[-1]["proved"]})
+                                   "prove": args.prove, "primes_proved_per_step": tims[-1]["proved"],
+                                   "plain_engine": plain},
+                        {"roofline": roof, "cpu_baseline": cpu})
 
 
 def bench_c4(args):
@@ -187,13 +254,28 @@ def bench_c4(args):
     ms = (time.perf_counter() - t0) / args.steps * 1e3
     tm = eng.last_timing()
     eng.close()
+    # k_psp dominates: W for a base-2 strong test of a 64-bit n is 63 squarings of 9 products
+    # (two 32-bit limbs: 3 square, 2 quotient-digit and 4 modulus products) and the final REDC (6)
+    w_sprp = 63 * 9 + 6
+    roof = int_roofline("k_psp", tm["psp_tests"] * w_sprp, tm["psp_ms"], tm["launches"],
+                        "base-2 strong tests run (%d, the numbers the order sieve and the small-factor filter keep) "
+                        "x %d products each; HIP-event time of the k_psp launches" % (tm["psp_tests"], w_sprp))
+    roof["test_kernel"] = int_roofline("k_test<A_M28>", r["tests"] * wf_products(64), tm["test_ms"], tm["launches"],
+                                       "tests x W(64 bits) = %d" % wf_products(64))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        oracle.scan((1 << 64) - 2, 1 << 64, bases=[2])       # the sieving primes below 2^32, once
+        cpu = cpu_sample([((1 << 64) - s, 1 << 64, {"bases": [2], "psp": True}) for s in
+                          (10 ** 6, 4 * 10 ** 6, 16 * 10 ** 6, 64 * 10 ** 6, 256 * 10 ** 6)], 10.0)
     return _engine_line("primes Wieferich-tested/sec at p~2^64 with SPRP-2 flags", r["tests"] / (ms / 1e3),
                         "primes tested/s", ms, args,
                         {"workload": "[2^64-%d, 2^64), base 2, strong pseudoprimes to base 2 listed" % span,
                          "engine": "plain", "tests": r["tests"], "cks": "%016x" % r["cks"], "psp": r["psp"],
                          "bpsw_pseudoprimes": r["nbpsw"],
                          "odd_numbers_per_s": span / 2 / (ms / 1e3), "sieve_ms": tm["sieve_ms"],
-                         "test_ms": tm["test_ms"]})
+                         "test_ms": tm["test_ms"], "psp_ms": tm["psp_ms"], "psp_tests": tm["psp_tests"]},
+                        {"roofline": roof, "cpu_baseline": cpu})
 
 
 OEIS_BASES = [3, 5, 6, 7, 10, 12, 13, 14, 15, 17, 18, 19, 20, 22, 23, 26, 30]
@@ -313,12 +395,12 @@ def bench_c5(args):
         for _ in qs:
             t += rng.expovariate(rate)
             arrivals.append(t)
-        lat, live, nxt, done_tests, numbers = [], {}, 0, 0, 0
+        lat, live, nxt, done_tests, numbers, prods = [], {}, 0, 0, 0, 0
         t0 = time.perf_counter()
         while nxt < len(qs) or live:
             now = time.perf_counter() - t0
             while nxt < len(qs) and arrivals[nxt] <= now:
-                tk = eng.serve_submit(*qs[nxt])
+                tk = eng.serve_submit(qs[nxt][0], qs[nxt][1], near_T=NEAR_T)
                 if tk is None:
                     break                                  # full: the request waits (its latency grows)
                 live[tk] = nxt
@@ -330,24 +412,36 @@ def bench_c5(args):
                     lat.append(time.perf_counter() - t0 - arrivals[i])
                     done_tests += r["tests"]
                     numbers += qs[i][1] - qs[i][0]
+                    prods += r["tests"] * wf_products(((qs[i][0] + qs[i][1]) // 2).bit_length())
                     if len(lat) % 2000 == 0:
                         print("rate %g: %d queries done, %.1f s" % (rate, len(lat), time.perf_counter() - t0),
                               file=sys.stderr, flush=True)
             time.sleep(0.0005)
         wall = time.perf_counter() - t0
         eng.serve_stop()
+        tm = eng.last_timing()
         lat.sort()
         sweep.append({"rate_per_s": rate, "queries": len(lat), "p50_s": lat[len(lat) // 2],
                       "p99_s": lat[min(len(lat) - 1, int(len(lat) * 0.99))], "wall_s": wall,
-                      "primes_tested_per_s": done_tests / wall, "numbers_per_s": numbers / wall})
+                      "primes_tested_per_s": done_tests / wall, "numbers_per_s": numbers / wall,
+                      "test_ms": tm["test_ms"], "sieve_ms": tm["sieve_ms"], "test_launches": tm["launches"],
+                      "products": prods})
         print("rate %g: %s" % (rate, sweep[-1]), file=sys.stderr, flush=True)
     eng.close()
     best = max(x["primes_tested_per_s"] for x in sweep)
+    top = max(sweep, key=lambda x: x["primes_tested_per_s"])
+    roof = int_roofline("k_test<A_M28> (serving steps)", top["products"], top["test_ms"], top["test_launches"],
+                        "at %g queries/s: sum over queries of tests x W(bit length of the query's middle); HIP-event "
+                        "time of the steps' test launches" % top["rate_per_s"])
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_sample([(lo, hi, {"bases": [2], "near_T": NEAR_T}) for lo, hi in qs], 15.0)
     return _engine_line("serving: primes Wieferich-tested/sec and p50/p99 latency", best, "primes tested/s",
                         sweep[-1]["wall_s"] * 1e3, args,
                         {"workload": "%d queries per rate, lengths logU[1e5,1e10], starts U[1e9,1e18], Poisson "
                                      "arrivals" % len(qs), "engine": "serving (continuous batching)",
-                         "prove": args.prove, "sweep": sweep})
+                         "prove": args.prove, "sweep": sweep},
+                        {"roofline": roof, "cpu_baseline": cpu})
 
 
 def main():
@@ -357,6 +451,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--arith", type=int, default=0, help="1 = force the 32-bit Montgomery kernels")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="c3: skip the plain-engine run (and the roofline)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
     ap.add_argument("--seg-bytes", type=int, default=0, help="c2/c3 with --no-tune: LDS segment bytes")
     ap.add_argument("--sieve-block", type=int, default=0, help="c2/c3 with --no-tune: threads per sieve workgroup")

@@ -106,7 +106,14 @@ typedef struct {
     uint64_t fault_p;       /* testing only, 0 = off: the residue of this p is taken as r + 1,
                                which fails the FLT check (exercises the FLT-ERROR path: the
                                reference's finish_lanes, wieferich.c:1058-1063) */
+    uint32_t pipeline;      /* wf_scan_batch: WF_PIPELINE_GRAPH (all passes in one HIP graph, the
+                               sieve of pass k + 1 beside the tests of pass k) or WF_PIPELINE_OFF
+                               (the passes one after another on one stream, each kernel timed:
+                               the plain engine, as wf_scan runs a range) */
+    uint32_t pad;
 } wf_config;
+
+enum { WF_PIPELINE_GRAPH = 0, WF_PIPELINE_OFF = 1 };
 
 /* Prove mode: a piece is sieved by the primes below bucket_min only, and its survivors are
  * decided by BPSW (base-2 strong test + strong Lucas test, exact below 2^64) instead of
@@ -217,12 +224,16 @@ int  wf_autotune(wf_engine *e, uint64_t lo, uint64_t hi, wf_config *best, double
  * order, up to 3e9 numbers, sieved as one packed pass. */
 int  wf_autotune_batch(wf_engine *e, const wf_range *r, uint32_t n, wf_config *best, double *best_ms);
 
-/* Timings of the last wf_scan / wf_scan_batch: device milliseconds of the sieve and
- * the test kernels (HIP events on the engine's stream), primes tested, device passes, the
- * host milliseconds spent planning the passes (segments and fill work items), and the
- * primes that came through prove mode. */
+/* Timings of the last wf_scan / wf_scan_batch, or of the serving session ended by the last
+ * wf_serve_stop: device milliseconds of the sieve and the test kernels (HIP events on the
+ * streams they run on; a pipelined batch times its whole graph only, total_ms), primes
+ * tested, test launches, the host milliseconds spent planning the passes, the primes that
+ * came through prove mode, and the strong-pseudoprime kernel's time and test count. */
 typedef struct { double sieve_ms, test_ms, total_ms; uint64_t tested; uint64_t launches; double plan_ms;
-                 uint64_t proved;   /* primes found by the prove-mode path */ } wf_timing;
+                 uint64_t proved;   /* primes found by the prove-mode path */
+                 double psp_ms;     /* the strong-pseudoprime kernel (wf_scan with want_psp) */
+                 uint64_t psp_tests;   /* base-2 strong tests it ran (numbers its filters kept) */
+               } wf_timing;
 int  wf_last_timing(wf_engine *e, wf_timing *t);
 
 #ifdef __cplusplus

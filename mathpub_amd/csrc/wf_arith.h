@@ -729,43 +729,80 @@ __device__ __forceinline__ u64 mont64_dbl(u64 x, u64 n) {
   return (c || y >= n) ? y - n : y;
 }
 
+// m^-1 mod 2^32 for odd m (Newton's iteration, exact after five steps)
+constexpr u32 inv32c(u32 m) {
+  u32 x = m;
+  for (int k = 0; k < 5; k++) x *= 2u - m * x;
+  return x;
+}
+
+// m | x for an odd constant m and any x < 2^32: x m^-1 mod 2^32 equals x / m exactly when m
+// divides x, and every other value exceeds (2^32 - 1) / m.  One multiply and one compare.
+template <u32 M>
+__device__ __forceinline__ bool divides32(u32 x) {
+  constexpr u32 I = inv32c(M);
+  return x * I <= 0xFFFFFFFFu / M;
+}
+
 // A necessary condition for an odd composite n to be a base-2 strong probable prime.
 // Write n - 1 = d 2^s.  Either 2^d = 1 (mod n), and then ord_r(2) | d for every prime factor
 // r of n, or 2^(d 2^t) = -1 (mod n) for some t < s, and then ord_r(2) | d 2^(t+1) but not
 // d 2^t.  Either way ord_r(2) | n - 1 and all prime factors share the 2-adic valuation of
-// ord_r(2).  Checked for the factors r < 100 (products of them are reduced first, so each r
-// costs one 32-bit remainder); false means n is certainly not a strong probable prime.
+// ord_r(2).  Checked for the factors r < 100; false means n is certainly not a strong
+// probable prime.
+// Branch-free: the residues of n modulo five products of the small primes (each below
+// 2^32) give, by divides32, both the prime factors r < 100 of n and the congruences n = 1
+// mod the odd prime powers in ord_r(2); those modulo 4, 8 and 16 are n's low bits.  The
+// 2-adic valuations of the orders of the factors found are collected as bits of vm, which
+// must hold at most one bit.  Every lane runs the same instructions, so a wavefront never
+// splits here.
 __device__ __forceinline__ bool psp_possible(u64 n) {
-  int v = -1;
+  constexpr u32 M1 = 9u * 5 * 7 * 11 * 13 * 17 * 19 * 23;   // 334639305
+  constexpr u32 M2 = 29u * 31 * 37 * 41 * 43;               // 58642669
+  constexpr u32 M3 = 47u * 53 * 59 * 61 * 67;               // 600662303
+  constexpr u32 M4 = 71u * 73 * 79 * 83 * 89;               // 3024658859
+  const u32 g1 = (u32)(n % M1), g2 = (u32)(n % M2), g3 = (u32)(n % M3), g4 = (u32)(n % M4);
+  const u32 g5 = (u32)(n % 97u);
+  // n = 1 modulo 4, 8, 16 and modulo odd divisors of M1 and M2 (g + M - 1 < 2^32 avoids the
+  // wrap of g - 1 at g = 0)
+  const u32 n4 = (u32)n & 15u;
+  const bool e4 = (n4 & 3u) == 1u, e8 = (n4 & 7u) == 1u, e16 = n4 == 1u;
+  const u32 h1 = g1 + (M1 - 1u), h2 = g2 + (M2 - 1u);
+  const bool e3 = divides32<3>(h1), e9 = divides32<9>(h1), e5 = divides32<5>(h1), e7 = divides32<7>(h1),
+             e11 = divides32<11>(h1), e13 = divides32<13>(h1), e23 = divides32<23>(h1), e29 = divides32<29>(h2),
+             e41 = divides32<41>(h2);
   bool ok = true;
-#define PSP_R(g, r, ord, v2)                                     \
-  if ((g) % (r##u) == 0u) {                                      \
-    if (n % (ord##u) != 1u || (v >= 0 && v != (v2))) ok = false; \
-    v = (v2);                                                    \
-  }
-  {
-    const u32 g = (u32)(n % 111546435u);   // 3 5 7 11 13 17 19 23
-    PSP_R(g, 3, 2, 1) PSP_R(g, 5, 4, 2) PSP_R(g, 7, 3, 0) PSP_R(g, 11, 10, 1)
-    PSP_R(g, 13, 12, 2) PSP_R(g, 17, 8, 3) PSP_R(g, 19, 18, 1) PSP_R(g, 23, 11, 0)
-  }
-  {
-    const u32 g = (u32)(n % 58642669u);    // 29 31 37 41 43
-    PSP_R(g, 29, 28, 2) PSP_R(g, 31, 5, 0) PSP_R(g, 37, 36, 2) PSP_R(g, 41, 20, 2) PSP_R(g, 43, 14, 1)
-  }
-  {
-    const u32 g = (u32)(n % 600662303u);   // 47 53 59 61 67
-    PSP_R(g, 47, 23, 0) PSP_R(g, 53, 52, 2) PSP_R(g, 59, 58, 1) PSP_R(g, 61, 60, 2) PSP_R(g, 67, 66, 1)
-  }
-  {
-    const u32 g = (u32)(n % 33984931u);    // 71 73 79 83
-    PSP_R(g, 71, 35, 0) PSP_R(g, 73, 9, 0) PSP_R(g, 79, 39, 0) PSP_R(g, 83, 82, 1)
-  }
-  {
-    const u32 g = (u32)(n % 8633u);        // 89 97
-    PSP_R(g, 89, 11, 0) PSP_R(g, 97, 48, 4)
-  }
-#undef PSP_R
-  return ok;
+  u32 vm = 0;
+  // r | n requires n = 1 (mod ord_r(2)); v = the 2-adic valuation of ord_r(2)
+  auto factor = [&](bool d, bool cond, int v) {
+    ok = ok && (!d || cond);
+    vm |= (d ? 1u : 0u) << v;
+  };
+  factor(divides32<3>(g1), true, 1);                 // ord 2
+  factor(divides32<5>(g1), e4, 2);                   // 4
+  factor(divides32<7>(g1), e3, 0);                   // 3
+  factor(divides32<11>(g1), e5, 1);                  // 10
+  factor(divides32<13>(g1), e4 && e3, 2);            // 12
+  factor(divides32<17>(g1), e8, 3);                  // 8
+  factor(divides32<19>(g1), e9, 1);                  // 18
+  factor(divides32<23>(g1), e11, 0);                 // 11
+  factor(divides32<29>(g2), e4 && e7, 2);            // 28
+  factor(divides32<31>(g2), e5, 0);                  // 5
+  factor(divides32<37>(g2), e4 && e9, 2);            // 36
+  factor(divides32<41>(g2), e4 && e5, 2);            // 20
+  factor(divides32<43>(g2), e7, 1);                  // 14
+  factor(divides32<47>(g3), e23, 0);                 // 23
+  factor(divides32<53>(g3), e4 && e13, 2);           // 52
+  factor(divides32<59>(g3), e29, 1);                 // 58
+  factor(divides32<61>(g3), e4 && e3 && e5, 2);      // 60
+  factor(divides32<67>(g3), e3 && e11, 1);           // 66
+  factor(divides32<71>(g4), e5 && e7, 0);            // 35
+  factor(divides32<73>(g4), e9, 0);                  // 9
+  factor(divides32<79>(g4), e3 && e13, 0);           // 39
+  factor(divides32<83>(g4), e41, 1);                 // 82
+  factor(divides32<89>(g4), e11, 0);                 // 11
+  factor(g5 == 0u, e16 && e3, 4);                    // 48
+  return ok && (vm & (vm - 1u)) == 0u;
 }
 
 // true iff n is a base-2 strong probable prime (n odd, n >= 3)

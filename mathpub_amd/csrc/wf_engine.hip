@@ -1099,7 +1099,7 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
                                              const u32* __restrict__ bitmap, u32 bmw, u32 tps,
                                              const PspOrder* __restrict__ po, u32 npo,
                                              u64* __restrict__ out, u64 cap, unsigned long long* __restrict__ count,
-                                             DevAcc* __restrict__ acc) {
+                                             DevAcc* __restrict__ acc, unsigned long long* __restrict__ ntests) {
   __shared__ u64 queue[PSP_TILE];
   __shared__ u32 ruled[PSP_TILE / 32];   // bit i: odd number i of the tile ruled out by the order sieve
   __shared__ u32 qn;
@@ -1147,6 +1147,7 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
     }
     __syncthreads();
     const u32 m = qn;
+    if (threadIdx.x == 0 && m) atomicAdd(ntests, (unsigned long long)m);
     auto record = [&](u64 n) {
       const unsigned long long kk = atomicAdd(count, 1ull);
       if (kk < cap) out[kk] = n;
@@ -1354,7 +1355,8 @@ struct wf_engine {
   DevEvent* d_ev = nullptr;
   u64 dev_ev_cap = 0;
   StatsOut* d_stats = nullptr;           // wf_sieve_stats totals
-  unsigned long long* d_ctr = nullptr;   // [1] events, [2] strong pseudoprimes, [3] proved primes
+  unsigned long long* d_ctr = nullptr;   // [1] events, [2] strong pseudoprimes, [3] proved primes,
+                                         // [4] strong tests run by k_psp
   u64* d_psp = nullptr;                  // strong pseudoprimes found
   u64 psp_cap = 0;
   PspOrder* d_porder = nullptr;          // the order sieve's primes (built on first use)
@@ -1515,7 +1517,7 @@ extern "C" int wf_open(int device, const wf_config* cfg, wf_engine** out) {
       e->cfg.seg_bytes > SEG_MAX || e->cfg.seg_bytes + 4096 > e->prop.sharedMemPerBlock || e->cfg.sieve_block < 64 ||
       e->cfg.sieve_block > 1024 || (e->cfg.sieve_block & 63) || e->cfg.test_block != 256 || e->cfg.arith > 1 ||
       e->cfg.bucket_min < 512 || e->cfg.chunk < 30ull * e->cfg.seg_bytes || e->cfg.prove > WF_PROVE_ON ||
-      e->cfg.derive > WF_DERIVE_OFF) {
+      e->cfg.derive > WF_DERIVE_OFF || e->cfg.pipeline > WF_PIPELINE_OFF) {
     delete e;
     return WF_E_ARG;
   }
@@ -2135,18 +2137,21 @@ static int run_pieces(wf_engine* e, const std::vector<Piece>& pieces, int arith,
     const u64 nt = P.segs.size() * (u64)tps;
     hipLaunchKernelGGL(k_psp, dim3((u32)std::min<u64>(nt, (u64)e->prop.multiProcessorCount * 16)), dim3(256), 0, e->st,
                        e->d_segs, (u32)P.segs.size(), e->bs[0].bitmap, e->cfg.seg_bytes / 4, tps, e->d_porder,
-                       e->nporder, e->d_psp, e->psp_cap, e->d_ctr + 2, to.acc);
+                       e->nporder, e->d_psp, e->psp_cap, e->d_ctr + 2, to.acc, e->d_ctr + 4);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e->ev[3], e->st));
   }
   u32 flags = 0;
   HIPCHK(hipMemcpyAsync(&flags, e->d_flags, sizeof flags, hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipStreamSynchronize(e->st));
   if (flags) return WF_E_OVERFLOW;
-  float a = 0, b = 0;
+  float a = 0, b = 0, c = 0;
   (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
   (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
+  if (want_psp) (void)hipEventElapsedTime(&c, e->ev[2], e->ev[3]);
   e->last.sieve_ms += a;
   e->last.test_ms += b;
+  e->last.psp_ms += c;
   e->last.launches += 1;
   return WF_OK;
 }
@@ -2298,7 +2303,7 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
   HIPCHK(hipMemsetAsync(e->d_acc, 0, n * sizeof(DevAcc), e->st));
   HIPCHK(hipMemsetAsync(e->d_cks, 0, ncks * sizeof(unsigned long long), e->st));
   HIPCHK(hipMemsetAsync(e->d_sumq, 0, ncks * sizeof(unsigned long long), e->st));
-  HIPCHK(hipMemsetAsync(e->d_ctr, 0, 4 * sizeof(unsigned long long), e->st));
+  HIPCHK(hipMemsetAsync(e->d_ctr, 0, 5 * sizeof(unsigned long long), e->st));
   if (want_psp && (rc = grow(e->d_psp, e->psp_cap, std::max<u64>(psp_cap, 1 << 16))) != WF_OK) return rc;
   HIPCHK(hipMemsetAsync(e->d_flags, 0, sizeof(u32), e->st));
   TestOut to;
@@ -2401,9 +2406,11 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
     res[i].npsp = acc[i].npsp;
     res[i].nbpsw = acc[i].nbpsw;
   }
-  unsigned long long proved = 0;
+  unsigned long long proved = 0, ptests = 0;
   HIPCHK(hipMemcpy(&proved, e->d_ctr + 3, sizeof proved, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&ptests, e->d_ctr + 4, sizeof ptests, hipMemcpyDeviceToHost));
   e->last.proved = proved;
+  e->last.psp_tests = ptests;
   if (want_psp) {
     unsigned long long np = 0;
     HIPCHK(hipMemcpy(&np, e->d_ctr + 2, sizeof np, hipMemcpyDeviceToHost));
@@ -2427,7 +2434,7 @@ extern "C" int wf_scan(wf_engine* e, u64 lo, u64 hi, int64_t near_T, wf_result* 
 extern "C" int wf_scan_batch(wf_engine* e, const wf_range* r, u32 n, int64_t near_T, wf_result* per_range,
                              wf_event* ev, u64 ev_cap) {
   if (!e || !r || !per_range || (ev_cap && !ev)) return WF_E_ARG;
-  return scan_ranges(e, r, n, near_T, per_range, ev, ev_cap, true);
+  return scan_ranges(e, r, n, near_T, per_range, ev, ev_cap, e->cfg.pipeline == WF_PIPELINE_GRAPH);
 }
 
 static int quotient_args(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_hi, u64* r_lo, u64 n) {
@@ -2784,6 +2791,7 @@ struct ServeStep {         // a step in flight
   unsigned long long* h_nev = nullptr;
   u32* h_flags = nullptr;
   hipEvent_t done = nullptr;
+  hipEvent_t tev[A_NCLS][4] = {};   // per pass (one per class): sieve start / end, test start / end
 };
 
 struct Server {
@@ -2808,6 +2816,7 @@ struct Server {
   u64 gpass = 0;                   // passes enqueued so far (their buffer sets alternate)
   u64 planned = 0;                 // steps planned
   u64 steps = 0;                   // steps finished
+  wf_timing tim = {};              // kernel times and tests of the session (wf_last_timing after stop)
 };
 
 // Wait for the oldest step in flight and hand its results to the requests.
@@ -2816,6 +2825,14 @@ static int serve_finish_oldest(wf_engine* e, Server* S) {
   S->inflight.erase(S->inflight.begin());
   if (hipEventSynchronize(c.done) != hipSuccess) return WF_E_DEVICE;
   if (*c.h_flags) return WF_E_OVERFLOW;   // a sieve bound was exceeded (bug guard)
+  for (size_t k = 0; k < c.plans.size(); k++) {
+    float a = 0, b = 0;
+    (void)hipEventElapsedTime(&a, c.tev[k][0], c.tev[k][1]);
+    (void)hipEventElapsedTime(&b, c.tev[k][2], c.tev[k][3]);
+    S->tim.sieve_ms += a;
+    S->tim.test_ms += b;
+    S->tim.launches++;
+  }
   const u64 nev = *c.h_nev;
   std::vector<DevEvent> ev(std::min<u64>(nev, SERVE_EVCAP));
   if (!ev.empty() && hipMemcpy(ev.data(), c.d_ev, ev.size() * sizeof(DevEvent), hipMemcpyDeviceToHost) != hipSuccess)
@@ -2853,6 +2870,7 @@ static int serve_finish_oldest(wf_engine* e, Server* S) {
     }
     std::sort(q.ev.begin(), q.ev.end(), cmp_dev_event);
     q.done = true;
+    S->tim.tested += q.res.tests;
     S->slot_ticket[k] = 0;
     S->active--;
   }
@@ -2976,12 +2994,16 @@ static int serve_enqueue(wf_engine* e, Server* S, int ci, const std::vector<Piec
     const int set = (int)((S->gpass + k) & 1);
     hipEvent_t sieved = e->gev[1 + set], tested = e->gev[3 + set];
     if (S->gpass + k >= 2) HIPCHK(hipStreamWaitEvent(e->st, tested, 0));
+    HIPCHK(hipEventRecord(c.tev[k][0], e->st));
     if ((rc = enqueue_sieve(e, P, c.d_segs + soff[k], c.d_runs + roff[k], c.d_fills + foff[k], e->bs[set], e->st,
                             S->d_acc)) != WF_OK)
       return rc;
+    HIPCHK(hipEventRecord(c.tev[k][1], e->st));
     HIPCHK(hipEventRecord(sieved, e->st));
     HIPCHK(hipStreamWaitEvent(e->st2, sieved, 0));
+    HIPCHK(hipEventRecord(c.tev[k][2], e->st2));
     if ((rc = enqueue_test(e, P, c.d_segs + soff[k], e->bs[set], e->st2, c.arith[k], to)) != WF_OK) return rc;
+    HIPCHK(hipEventRecord(c.tev[k][3], e->st2));
     HIPCHK(hipEventRecord(tested, e->st2));
   }
   S->gpass += c.plans.size();
@@ -3077,6 +3099,9 @@ static void serve_free(Server* S) {
     (void)hipHostFree(c.h_nev);
     (void)hipHostFree(c.h_flags);
     if (c.done) (void)hipEventDestroy(c.done);
+    for (auto& pass : c.tev)
+      for (hipEvent_t& x : pass)
+        if (x) (void)hipEventDestroy(x);
   }
   delete S;
 }
@@ -3111,6 +3136,9 @@ extern "C" int wf_serve_start(wf_engine* e, u32 slots, u64 hbm_budget) {
          hipHostMalloc(&c.h_nev, sizeof(unsigned long long), 0) == hipSuccess &&
          hipHostMalloc(&c.h_flags, sizeof(u32), 0) == hipSuccess &&
          hipEventCreateWithFlags(&c.done, hipEventDisableTiming) == hipSuccess;
+  for (ServeStep& c : S->step)
+    for (auto& pass : c.tev)
+      for (hipEvent_t& x : pass) ok = ok && hipEventCreate(&x) == hipSuccess;
   if (!ok) {
     serve_free(S);
     return WF_E_NOMEM;
@@ -3181,6 +3209,8 @@ extern "C" int wf_serve_stop(wf_engine* e) {
   S->cv.notify_all();
   if (S->th.joinable()) S->th.join();
   const int err = S->err;
+  e->last = S->tim;
+  e->last.total_ms = S->tim.sieve_ms + S->tim.test_ms;
   serve_free(S);
   e->srv = nullptr;
   return err;

@@ -12,6 +12,7 @@ LIB = os.path.join(PKG, "libwf.so")
 MAXB = 64
 EV_FOUND, EV_NEAR, EV_FLT = 0, 1, 2
 PROVE_AUTO, PROVE_OFF, PROVE_ON = 0, 1, 2
+PIPELINE_GRAPH, PIPELINE_OFF = 0, 1
 DERIVE_AUTO, DERIVE_OFF = 0, 1     # composite bases from their prime factors' quotients
 BPSW_SPRP2, BPSW_LUCAS, BPSW_PRIME = 1, 2, 4
 STATIC_QUOT_MAX = 1 << 18          # wf_quotients_static's fixed scratch area (values per call)
@@ -43,12 +44,14 @@ class Range(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("seg_bytes", u32), ("sieve_block", u32), ("test_block", u32), ("arith", u32), ("bucket_min", u32),
-                ("chunk", u64), ("test_wgs", u32), ("prove", u32), ("derive", u32), ("fault_step", u32), ("fault_p", u64)]
+                ("chunk", u64), ("test_wgs", u32), ("prove", u32), ("derive", u32), ("fault_step", u32), ("fault_p", u64),
+                ("pipeline", u32), ("pad", u32)]
 
 
 class Timing(ctypes.Structure):
     _fields_ = [("sieve_ms", ctypes.c_double), ("test_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("tested", u64), ("launches", u64), ("plan_ms", ctypes.c_double), ("proved", u64)]
+                ("tested", u64), ("launches", u64), ("plan_ms", ctypes.c_double), ("proved", u64),
+                ("psp_ms", ctypes.c_double), ("psp_tests", u64)]
 
 
 class WFError(RuntimeError):
@@ -129,12 +132,13 @@ class Engine:
     """One GPU engine (wf_open); not thread-safe, like the C object."""
 
     def __init__(self, device=0, seg_bytes=0, sieve_block=0, arith=0, chunk=0, bucket_min=0, test_wgs=0,
-                 prove=PROVE_AUTO, derive=DERIVE_AUTO, fault_p=0, fault_step=0):
+                 prove=PROVE_AUTO, derive=DERIVE_AUTO, fault_p=0, fault_step=0, pipeline=PIPELINE_GRAPH):
         """fault_p and fault_step are for testing only: the residue of p = fault_p is taken as
         r + 1, so its pair fails the FLT check and is reported as an FLT event; serving step
         number fault_step fails (every request then ends with WF_E_DEVICE)."""
         L = load()
-        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs, prove, derive, fault_step, fault_p)
+        cfg = Config(seg_bytes, sieve_block, 0, arith, bucket_min, chunk, test_wgs, prove, derive, fault_step, fault_p,
+                     pipeline, 0)
         h = ctypes.c_void_p()
         _check(L.wf_open(device, ctypes.byref(cfg), ctypes.byref(h)), "wf_open")
         self._h = h
@@ -285,4 +289,5 @@ class Engine:
         t = Timing()
         _check(self._L.wf_last_timing(self._h, ctypes.byref(t)), "wf_last_timing")
         return {"sieve_ms": t.sieve_ms, "test_ms": t.test_ms, "total_ms": t.total_ms, "tested": t.tested,
-                "launches": t.launches, "plan_ms": t.plan_ms, "proved": t.proved}
+                "launches": t.launches, "plan_ms": t.plan_ms, "proved": t.proved, "psp_ms": t.psp_ms,
+                "psp_tests": t.psp_tests}

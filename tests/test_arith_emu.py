@@ -182,6 +182,52 @@ def test_psp_filter_keeps_every_strong_pseudoprime(emu):
     assert 0.4 < rejected / comps < 0.8
 
 
+def _order2(r):
+    k, x = 1, 2 % r
+    while x != 1:
+        x = x * 2 % r
+        k += 1
+    return k
+
+
+def _v2(x):
+    return (x & -x).bit_length() - 1
+
+
+def psp_possible_definition(n):
+    """For every prime r < 100 dividing n: ord_r(2) divides n - 1, and the 2-adic valuations
+    of these orders are all equal."""
+    vs = set()
+    for r in (3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97):
+        if n % r == 0:
+            o = _order2(r)
+            if (n - 1) % o:
+                return False
+            vs.add(_v2(o))
+    return len(vs) <= 1
+
+
+def test_psp_filter_equals_its_definition(emu):
+    """The branch-free filter against the condition written out with Python integers, on
+    random odd numbers up to 2^64 and on products of small primes (which exercise many
+    factors and every valuation at once)."""
+    rng = random.Random(11)
+    small = [3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97]
+    ns = [rng.randrange(3, 1 << 64) | 1 for _ in range(40000)] + list(range(3, 60000, 2))
+    for _ in range(40000):
+        m = 1
+        for r in rng.sample(small, rng.randint(1, 4)):
+            m *= r ** rng.randint(1, 2)
+        k = rng.randrange(1, max(2, ((1 << 64) - 1) // m))
+        n = m * k
+        if n % 2 == 0:
+            n += m if m % 2 else 0
+        if n % 2 and 3 <= n < 1 << 64:
+            ns.append(n)
+    bad = [n for n in ns if bool(emu.emu_psp_possible(n)) != psp_possible_definition(n)]
+    assert bad == []
+
+
 def _isprime(n):
     if n < 2:
         return False
