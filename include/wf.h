@@ -51,7 +51,7 @@ enum {
     WF_E_ARG = -1,          /* bad argument (range, sizes, NULL pointer) */
     WF_E_DEVICE = -2,       /* no GPU, or a HIP runtime call failed */
     WF_E_NOMEM = -3,        /* device or host allocation failed */
-    WF_E_UNSUPPORTED = -4,  /* a base other than 2, or a feature not built */
+    WF_E_UNSUPPORTED = -4,  /* a feature the device cannot run (segments above its LDS) */
     WF_E_OVERFLOW = -5,     /* an internal buffer bound was exceeded (bug guard) */
     WF_E_BUSY = -6,         /* serving: no free slot / HBM budget; or a call that needs the
                                engine (scans, quotients, BPSW, statistics, autotuning,
@@ -136,8 +136,9 @@ int  wf_info(wf_engine *e, char *buf, int len);
  * parse_bases, wieferich.c:816-849).  Results carry one checksum and one sum of q per base
  * (index j = position in this list); pairs with p | b are skipped, as process_batch does
  * (wieferich.c:1097).  Base 2 runs its own ladders; any other base uses windows of w bits
- * with multipliers b^d < 2^32 (wieferich.c:99-107).  wf_config.arith = 1 supports base 2
- * only (WF_E_UNSUPPORTED otherwise).  The default is the single base 2. */
+ * with multipliers b^d < 2^32 (wieferich.c:99-107).  wf_config.arith = 1 computes every base
+ * with 32-bit limb Montgomery ladders: doublings for base 2, the binary ladder (a product by
+ * b R mod p^2 per set bit) for the others.  The default is the single base 2. */
 int  wf_set_bases(wf_engine *e, const uint32_t *bases, int nb);
 
 /* Scan one range: totals into *out, events sorted by (p, j) into ev[0..ev_cap).

@@ -294,6 +294,42 @@ __device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64
 // 32-bit limb Montgomery path
 // ---------------------------------------------------------------------------------
 
+// REDC of a double-width t < N R: x = t / R mod N, one low word cleared per row.  The
+// result is below 2N; FULL subtracts N once (keeping the carry bit past 2^(32L)).
+template <int L, bool FULL>
+__device__ __forceinline__ void mont_redc(u32 (&t)[2 * L], u32 (&x)[L], const u32 (&n)[L], u32 ninv) {
+  u32 hi = 0;   // the carry pending into word i + L + 1
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    const u32 mq = t[i] * ninv;
+    u64 c = 0;
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+      c = (u64)mq * n[j] + t[i + j] + (c >> 32);
+      t[i + j] = (u32)c;
+    }
+    const u64 s = (u64)t[i + L] + (c >> 32) + hi;
+    t[i + L] = (u32)s;
+    hi = (u32)(s >> 32);
+  }
+#pragma unroll
+  for (int i = 0; i < L; i++) x[i] = t[L + i];
+  if (FULL) {
+    // result = hi 2^(32L) + x < 2N: subtract N once if it is >= N
+    u32 d[L];
+    u64 br = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+      const u64 s = (u64)x[i] - n[i] - br;
+      d[i] = (u32)s;
+      br = (s >> 32) & 1;
+    }
+    const bool ge = hi || !br;
+#pragma unroll
+    for (int i = 0; i < L; i++) x[i] = ge ? d[i] : x[i];
+  }
+}
+
 // Montgomery squaring t = x^2 / R mod N (R = 2^(32L)), ninv = -N^-1 mod 2^32.
 // Lazy (FULL = false): x < 2N, N < R/4 gives a result < 2N and no final subtraction.
 // FULL: x < N < R; the result (< 2N < 2^(32L+1)) keeps its carry bit and is reduced.
@@ -329,37 +365,26 @@ __device__ __forceinline__ void mont_sqr(u32 (&x)[L], const u32 (&n)[L], u32 nin
       t[2 * i + 1] = (u32)c;
     }
   }
-  // REDC: clear one low word per row; 'hi' is the carry pending into word i + L + 1
-  u32 hi = 0;
+  mont_redc<L, FULL>(t, x, n, ninv);
+}
+
+// x = x y / R mod N for x < 2N (lazy) or x < N (FULL) and y < N: the same bounds as mont_sqr.
+template <int L, bool FULL>
+__device__ __forceinline__ void mont_mul(u32 (&x)[L], const u32 (&y)[L], const u32 (&n)[L], u32 ninv) {
+  u32 t[2 * L];
+#pragma unroll
+  for (int k = 0; k < 2 * L; k++) t[k] = 0;
 #pragma unroll
   for (int i = 0; i < L; i++) {
-    const u32 mq = t[i] * ninv;
     u64 c = 0;
 #pragma unroll
     for (int j = 0; j < L; j++) {
-      c = (u64)mq * n[j] + t[i + j] + (c >> 32);
+      c = (u64)x[i] * y[j] + t[i + j] + (c >> 32);
       t[i + j] = (u32)c;
     }
-    const u64 s = (u64)t[i + L] + (c >> 32) + hi;
-    t[i + L] = (u32)s;
-    hi = (u32)(s >> 32);
+    t[i + L] = (u32)(c >> 32);   // row i is the first to write word i + L
   }
-#pragma unroll
-  for (int i = 0; i < L; i++) x[i] = t[L + i];
-  if (FULL) {
-    // result = hi 2^(32L) + x < 2N: subtract N once if it is >= N
-    u32 d[L];
-    u64 br = 0;
-#pragma unroll
-    for (int i = 0; i < L; i++) {
-      const u64 s = (u64)x[i] - n[i] - br;
-      d[i] = (u32)s;
-      br = (s >> 32) & 1;
-    }
-    const bool ge = hi || !br;
-#pragma unroll
-    for (int i = 0; i < L; i++) x[i] = ge ? d[i] : x[i];
-  }
+  mont_redc<L, FULL>(t, x, n, ninv);
 }
 
 // x = 2x mod M (M = 2N in the lazy domain [0, 2N), M = N for FULL with carry-out).
@@ -459,6 +484,43 @@ __device__ __forceinline__ void mont_pow2(u64 p, u64 e, int ebits, u32 (&r)[L]) 
     if (!lead) {
       mont_sqr<L, FULL>(x, n, ninv);
       if ((e >> i) & 1) mont_dbl<L>(x, M);
+    }
+  }
+  mont_out<L>(x, n, ninv);
+#pragma unroll
+  for (int i = 0; i < L; i++) r[i] = x[i];
+}
+
+// r = b^e mod N for any base b (N = p^2) by the plain binary ladder: one squaring per bit and
+// one Montgomery product by B = b R mod N per set bit.  This is a second arithmetic for the
+// bases other than 2, independent of the window ladders of the FP64 and radix-2^28 paths
+// (no power table, no window, 32-bit limbs): the double check recomputes with it.
+template <int L, bool FULL>
+__device__ __forceinline__ void mont_powb(u64 p, u64 e, int ebits, u32 b, u32 (&r)[L]) {
+  const u64 lo = p * p;
+  const u64 hi = __umul64hi(p, p);
+  u32 n[L];
+  {
+    const u32 w4[4] = {(u32)lo, (u32)(lo >> 32), (u32)hi, (u32)(hi >> 32)};
+#pragma unroll
+    for (int i = 0; i < L; i++) n[i] = i < 4 ? w4[i] : 0u;
+  }
+  const u64 pinv = inv64(p);
+  const u32 ninv = 0u - (u32)pinv * (u32)pinv;
+  // B = (b mod N) doubled 32L times mod N: every step keeps it below N
+  const u64 bm = hi || b < lo ? (u64)b : (u64)b % lo;
+  u32 B[L];
+#pragma unroll
+  for (int i = 0; i < L; i++) B[i] = i == 0 ? (u32)bm : i == 1 ? (u32)(bm >> 32) : 0u;
+  for (int k = 0; k < 32 * L; k++) mont_dbl<L>(B, n);
+  u32 x[L];
+#pragma unroll
+  for (int i = 0; i < L; i++) x[i] = B[i];
+  for (int i = ebits - 2; i >= 0; i--) {
+    const bool lead = i >= 63 - (int)__builtin_clzll(e);    // above this lane's top bit
+    if (!lead) {
+      mont_sqr<L, FULL>(x, n, ninv);
+      if ((e >> i) & 1) mont_mul<L, FULL>(x, B, n, ninv);
     }
   }
   mont_out<L>(x, n, ninv);

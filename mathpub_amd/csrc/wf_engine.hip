@@ -823,11 +823,12 @@ __device__ __forceinline__ void residue(u64 p, int eb, u32 b, int w, const u32* 
     rlo = (u64)x[0] | ((u64)x[1] << 32);
     rhi = (u64)x[2] | ((u64)x[3] << 32);
   } else {
-    // the 32-bit limb ladders implement base 2 (the host rejects other bases for them)
+    // the 32-bit limb ladders: base 2 by doublings, any other base by the binary ladder
     constexpr int L = ARITH == A_M2 ? 2 : (ARITH == A_M3 ? 3 : 4);
     constexpr bool FULL = ARITH == A_FULL;
     u32 x[L];
-    mont_pow2<L, FULL>(p, e, eb, x);
+    if (b == 2) mont_pow2<L, FULL>(p, e, eb, x);
+    else mont_powb<L, FULL>(p, e, eb, b, x);
     rlo = (u64)x[0] | ((u64)x[1] << 32);
     rhi = (L > 2 ? (u64)x[2] : 0) | (L > 3 ? ((u64)x[3] << 32) : 0);
   }
@@ -1058,7 +1059,7 @@ __global__ void __launch_bounds__(256) k_test_derived(const SegDesc* __restrict_
 
 // Quotients of arbitrary odd p (check / selftest) for the base in slot `slot` of the base
 // tables: per-lane choice of the path, the scan's classes (int32 = 1 selects the 32-bit
-// Montgomery ladders above 2^50, base 2 only).
+// Montgomery ladders above 2^50).
 __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u64* __restrict__ rhi_out,
                        u64* __restrict__ rlo_out, u64 n, int int32, u32 slot) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1378,7 +1379,7 @@ extern "C" const char* wf_strerror(int rc) {
     case WF_E_ARG: return "bad argument";
     case WF_E_DEVICE: return "GPU not available or HIP runtime error";
     case WF_E_NOMEM: return "out of memory";
-    case WF_E_UNSUPPORTED: return "not supported by the GPU engine (base 2 only)";
+    case WF_E_UNSUPPORTED: return "not supported by the GPU engine";
     case WF_E_OVERFLOW: return "internal buffer bound exceeded";
     case WF_E_BUSY: return "engine busy serving (no free slot, or wf_serve_stop not called yet)";
     case WF_E_LIMIT: return "more values than the fixed scratch area of wf_quotients_static holds";
@@ -2283,8 +2284,6 @@ static int scan_ranges(wf_engine* e, const wf_range* r, u32 n, i64 near_T, wf_re
                        u64 ev_cap, bool batched, bool want_psp = false, u64* psp = nullptr, u64 psp_cap = 0) {
   if (!e || !r || !res || n == 0 || (want_psp && n != 1)) return WF_E_ARG;
   if (e->srv) return WF_E_BUSY;   // the server thread owns the engine until wf_serve_stop
-  // the 32-bit limb ladders implement base 2 only
-  if (e->cfg.arith == 1 && (e->nb != 1 || e->bases[0] != 2)) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
   u64 maxtop = 0;
   for (u32 i = 0; i < n; i++) {
@@ -2442,7 +2441,6 @@ static int quotient_args(wf_engine* e, uint32_t b, const u64* p, u64* q, u64* r_
   if (e->srv) return WF_E_BUSY;
   for (u64 i = 0; i < n; i++)
     if (p[i] < 3 || !(p[i] & 1)) return WF_E_ARG;
-  if (e->cfg.arith == 1 && b != 2) return WF_E_UNSUPPORTED;
   return WF_OK;
 }
 
@@ -3108,7 +3106,6 @@ static void serve_free(Server* S) {
 
 extern "C" int wf_serve_start(wf_engine* e, u32 slots, u64 hbm_budget) {
   if (!e || slots == 0 || slots > 65536 || e->srv) return WF_E_ARG;
-  if (e->cfg.arith == 1 && (e->nb != 1 || e->bases[0] != 2)) return WF_E_UNSUPPORTED;
   HIPCHK(hipSetDevice(e->dev));
   Server* S = new Server();
   S->slots = slots;

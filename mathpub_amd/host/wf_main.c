@@ -964,8 +964,9 @@ static bool already_checked(const char *out, u64 lo, const char *hi)
  * the GPU.  Every STEP-th line (default 16) of a chunk log is recomputed by an engine set up
  * differently from a scan's: prove mode for every piece (small-prime sieve + BPSW instead of
  * the bucket sieve), 16 KiB segments, one ladder per base (a scan derives composite bases
- * from their prime factors), and for the single base 2 the 32-bit limb ladders instead of the
- * FP64 p-adic / radix-2^28 arithmetic.  The recomputed line must equal the logged one byte
+ * from their prime factors), and the 32-bit limb Montgomery ladders (doublings for base 2, the
+ * plain binary ladder for any other base) instead of the FP64 p-adic / radix-2^28 window
+ * ladders.  The recomputed line must equal the logged one byte
  * for byte: "MATCH lo hi TIME" or "MISMATCH lo hi gpu: LINE check: LINE" is appended to OUT
  * (default CHUNKLOG.dcheck).  Chunks already in OUT are skipped, so a run resumes; with PID,
  * new lines are awaited while that process lives (dcheck.sh:12-45).  THREADS is accepted and
@@ -993,7 +994,7 @@ static int cmd_dcheck(int argc, char **argv)
     g_conf.prove = WF_PROVE_ON;
     g_conf.seg_bytes = 16384;
     g_conf.derive = WF_DERIVE_OFF;
-    if (g_nbase == 1 && g_base[0] == 2) g_conf.arith = 1;
+    g_conf.arith = 1;
     open_gpu();
     FILE *lf = fopen(logp, "r");
     if (!lf) fail("cannot open %s: %s", logp, strerror(errno));

@@ -56,6 +56,18 @@ void emu_mont(int L, int full, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo
   *hi = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
 }
 
+// any base b through the L-limb binary ladder (mont_powb)
+void emu_montb(int L, int full, uint32_t b, uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
+  const uint64_t e = p - 1;
+  uint32_t x[4] = {0, 0, 0, 0};
+  if (L == 2) { uint32_t y[2]; mont_powb<2, false>(p, e, ebits, b, y); x[0] = y[0]; x[1] = y[1]; }
+  else if (L == 3) { uint32_t y[3]; mont_powb<3, false>(p, e, ebits, b, y); x[0] = y[0]; x[1] = y[1]; x[2] = y[2]; }
+  else if (!full) { mont_powb<4, false>(p, e, ebits, b, x); }
+  else { mont_powb<4, true>(p, e, ebits, b, x); }
+  *lo = (uint64_t)x[0] | ((uint64_t)x[1] << 32);
+  *hi = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
+}
+
 
 // window and powers of a base, as the engine sets them up: the largest w <= 5 with
 // b^(2^w - 1) < 2^32, pw[d] = b^d

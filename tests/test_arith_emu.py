@@ -23,6 +23,7 @@ def emu():
     P = ctypes.POINTER(ctypes.c_uint64)
     L.emu_fp64.argtypes = [ctypes.c_uint64, P, P]
     L.emu_mont.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, P, P]
+    L.emu_montb.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, P, P]
     L.emu_sprp2.argtypes = [ctypes.c_uint64]
     L.emu_psp_possible.argtypes = [ctypes.c_uint64]
     L.emu_m28.argtypes = [ctypes.c_uint64, ctypes.c_int, P, P]
@@ -120,6 +121,28 @@ def test_montgomery(emu, L, full, lo, hi):
         assert call(emu.emu_mont, L, full, p, eb) == pow(2, p - 1, p * p), (L, p)
         # a longer wave-wide exponent length only adds skipped leading bits
         assert call(emu.emu_mont, L, full, p, min(64, eb + 3)) == pow(2, p - 1, p * p), (L, p)
+
+
+@pytest.mark.parametrize("L,full,lo,hi", [(2, 0, 2, 31), (3, 0, 32, 47), (4, 0, 48, 63), (4, 1, 64, 64),
+                                          (4, 1, 63, 64)])
+def test_montgomery_any_base(emu, L, full, lo, hi):
+    """The binary ladder of the 32-bit limb classes for bases other than 2, against Python's
+    pow: bases up to 2^32 - 1, bases at or above p^2 (p = 3, 5, 7), and p dividing b."""
+    ps = samples(lo, hi, 600, L * 10 + full + 100)
+    if L == 4 and full:
+        ps += [(1 << 64) - 1, (1 << 64) - 59, (1 << 63) + 1]
+    if L == 2:
+        ps += [3, 5, 7, 9, 11, 13, 65537, (1 << 31) - 1]
+    rng = random.Random(L * 7 + full)
+    bases = [3, 5, 6, 7, 10, 30, 97, 1625, 65537, 4294967291, 4294967295] + [rng.randrange(3, 1 << 32) for _ in range(4)]
+    for p in ps:
+        if p < 3:
+            continue
+        eb = (p - 1).bit_length()
+        for b in bases:
+            want = pow(b, p - 1, p * p)
+            assert call(emu.emu_montb, L, full, b, p, eb) == want, (L, b, p)
+            assert call(emu.emu_montb, L, full, b, p, min(64, eb + 2)) == want, (L, b, p)
 
 
 def strong2(n):

@@ -264,8 +264,11 @@ def test_dcheck_matches_then_flags_a_bad_line(tmp_path):
 
 @pytest.mark.gpu
 def test_dcheck_other_bases_and_top_of_word(tmp_path):
-    """The 'oeis' base list at 1e12 and base 2 just below 2^64 (END = 2^64 in the log)."""
-    for start, end, chunk, bases in (("1e12", "1e12+4e6", "2e6", "oeis"), ("2^64-4e6", "2^64", "2e6", "2")):
+    """The 'oeis' base list at 1e12 (the 32-bit binary ladder against the scan's FP64 windows),
+    prime bases across 2^63 (against the radix-2^28 windows), and base 2 just below 2^64
+    (END = 2^64 in the log)."""
+    for start, end, chunk, bases in (("1e12", "1e12+4e6", "2e6", "oeis"), ("2^63-2e6", "2^63+2e6", "2e6", "3,5,7,11,13"),
+                                     ("2^64-4e6", "2^64", "2e6", "2")):
         log = tmp_path / ("log" + bases)
         rc, out, err = run("scan", start, end, "-c", chunk, "-b", bases, "-L", log, "-q")
         assert rc == 0, err

@@ -6,6 +6,7 @@ import json
 import os
 import random
 import sys
+import time
 
 import pytest
 
@@ -239,11 +240,24 @@ def test_tiny_and_empty_ranges(eng):
 OEIS = FACTS["oeis_bases"]
 
 
-@pytest.fixture(scope="module")
-def feng():
-    e = wf.Engine()
+@pytest.fixture(scope="module", params=[0, 1], ids=["fast", "int32"])
+def feng(request):
+    """The fast paths (FP64 / radix-2^28 window ladders), and the 32-bit limb binary ladder,
+    the double check's independent arithmetic for bases other than 2."""
+    e = wf.Engine(arith=request.param)
     yield e
     e.close()
+
+
+_ORACLE = {}
+
+
+def oracle_scan(lo, hi, bases, near_T):
+    """oracle.scan, once per argument set for both engines of feng."""
+    k = (lo, hi, tuple(bases), near_T)
+    if k not in _ORACLE:
+        _ORACLE[k] = oracle.scan(lo, hi, bases=bases, near_T=near_T)
+    return _ORACLE[k]
 
 
 @pytest.mark.parametrize("w", range(3))
@@ -281,7 +295,7 @@ def test_any_bases_vs_oracle(feng, lo, hi):
     feng.set_bases(bases)
     say("bases vs oracle", lo, hi)
     r = feng.scan(lo, hi, near_T=100)
-    o = oracle.scan(lo, hi, bases=bases, near_T=100)
+    o = oracle_scan(lo, hi, bases, 100)
     assert (r["primes"], r["tests"], r["near"], r["flt"]) == (o["primes"], o["tests"], o["near"], o["flt"])
     assert r["cks_all"] == [o["cks"][b] for b in bases]
     assert r["sumq_all"] == [o["sumq"][b] for b in bases]
@@ -343,12 +357,34 @@ def test_composite_bases_derived_equal_direct(bases):
         off.close()
 
 
-def test_int32_path_is_base_2_only():
-    e = wf.Engine(arith=1)
-    try:
-        e.set_bases([2, 3])
-        with pytest.raises(wf.WFError) as ex:
-            e.scan(10 ** 12, 10 ** 12 + 1000)
-        assert ex.value.rc == -4
-    finally:
-        e.close()
+def test_int32_any_base_batch_and_serving():
+    """The 32-bit limb ladders for bases other than 2 through wf_scan_batch and serving, against
+    the fast paths: every class (2, 3 and 4 limbs, and the 4 limbs with a carry bit above 2^63)."""
+    bases = [3, 5, 12, 4294967291]
+    rng = random.Random(41)
+    ranges = [(3, 10 ** 5), ((1 << 31) - 10 ** 5, (1 << 31) + 10 ** 5), ((1 << 47) - 10 ** 5, (1 << 47) + 10 ** 5),
+              ((1 << 63) - 10 ** 5, (1 << 63) + 10 ** 5), ((1 << 64) - 10 ** 5, 1 << 64)]
+    ranges += [(lo, lo + rng.randrange(1, 10 ** 5)) for lo in (rng.randrange(1, 1 << 64) for _ in range(20))]
+    ranges = [(lo, hi if hi < 1 << 64 else 0) for lo, hi in ranges]
+    with wf.Engine() as fast, wf.Engine(arith=1) as m32:
+        for e in (fast, m32):
+            e.set_bases(bases)
+        pa, ea = fast.scan_batch(ranges, near_T=10 ** 4)
+        pb, eb = m32.scan_batch(ranges, near_T=10 ** 4)
+        m32.serve_start(slots=8, hbm_budget=2 << 30)
+        tickets = [m32.serve_submit(lo, hi or 1 << 64, near_T=10 ** 4) for lo, hi in ranges[:6]]
+        served = {}
+        t0 = time.time()
+        while len(served) < len(tickets) and time.time() - t0 < 60:
+            for t in tickets:
+                if t not in served:
+                    r = m32.serve_poll(t)
+                    if r is not None:
+                        served[t] = r
+            time.sleep(0.002)
+        m32.serve_stop()
+    keys = ("primes", "tests", "near", "flt", "cks_all", "sumq_all", "nev")
+    assert [[x[k] for k in keys] for x in pa] == [[x[k] for k in keys] for x in pb]
+    assert ea == eb
+    for i, t in enumerate(tickets):
+        assert [served[t][k] for k in keys] == [pa[i][k] for k in keys], ranges[i]
