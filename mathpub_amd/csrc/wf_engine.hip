@@ -264,6 +264,30 @@ __device__ __forceinline__ bool first_offset(u64 base, u32 q, u64 minv, u64 span
   return o < span;
 }
 
+// The same for a base below 2^53 (exact as a double), from rq = 1/q rounded to nearest:
+// t = floor(base * rq) is within one of floor(base / q) (the relative errors of rq and of the
+// product are below 2^-52 each, and base / q < 2^53 / q), so the remainder base - t q, which
+// one FMA gives exactly (an integer below 2q in magnitude), needs one correction each way.
+// chk_sq (uniform): q^2 may lie at or above the base.
+__device__ __forceinline__ bool first_offset_fp(double bd, u64 base, u32 q, double rq, u64 span, bool chk_sq,
+                                                u64& o) {
+  if (chk_sq) {
+    const u64 qq = (u64)q * q;
+    if (qq >= base) {
+      o = qq - base;
+      return o < span;
+    }
+  }
+  const double qd = (double)q;
+  const double t = __builtin_floor(bd * rq);
+  double r = __builtin_fma(-t, qd, bd);
+  r = r < 0.0 ? r + qd : r;
+  r = r >= qd ? r - qd : r;
+  const u32 ri = (u32)r;
+  o = ri ? q - ri : 0;
+  return o < span;
+}
+
 // Every hit of an item's primes in its fill group: hit(segment of the group, byte offset
 // within the run, class), or hit4 for four hits of one class at once (their LDS atomics are
 // then in flight together instead of each waiting for the last).  A lane takes one prime
@@ -416,7 +440,7 @@ __device__ __forceinline__ u32 word_mask(u32 w, u32 lo_off, u32 hi_off) {
 __attribute__((amdgpu_waves_per_eu(SIEVE_WAVES_EU)))
 __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs, u32 nseg,
                                                 const u32* __restrict__ sp, const u64* __restrict__ spinv,
-                                                u32 tierA, u32 q1idx,
+                                                const double* __restrict__ sprcp, u32 tierA, u32 q1idx,
                                                 const u32* __restrict__ bcnt, const u32* __restrict__ bdata,
                                                 u32 bcap,
                                                 u32* __restrict__ out, u32* __restrict__ cnt, u32 pcap,
@@ -437,33 +461,56 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
     const u64 span = 30ull * d.nbytes;
     // start from the pattern of the presieve primes 7, 11, 13, 17, 19 (period 323323 bytes)
     {
+      // four loads in flight per lane before the LDS writes
       const u32 ph = (u32)((d.base / 30) % PRE_PERIOD);
       const u32* pw = pre + (u64)(ph & 3) * pre_words + (ph >> 2);
-      for (u32 w = threadIdx.x; w < nwords; w += blockDim.x) seg[w] = pw[w];
+      const u32 bd = blockDim.x;
+      for (u32 w = threadIdx.x; w < nwords; w += 4 * bd) {
+        u32 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = w + k * bd < nwords ? pw[w + k * bd] : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (w + k * bd < nwords) seg[w + k * bd] = v[k];
+      }
     }
     __syncthreads();
     if (d.base == 0 && threadIdx.x == 0) seg[0] &= ~0x3Eu;   // 7, 11, 13, 17, 19 themselves are prime
     __syncthreads();
     const u32 nsmall = min(d.nq, q1idx);
-    // tier A: one wavefront per prime, lanes stride over the marks of each class
+    // Placement (uniform per segment): below 2^53 from the FP64 reciprocal table, else by
+    // Barrett reduction.  The q^2 test is needed only if the largest prime applied reaches it.
+    const bool fp = d.base < (1ull << 53);
+    const double bd = (double)d.base;
+    const u64 qtop = nsmall ? sp[nsmall - 1] : 0;
+    const bool chk_sq = qtop * qtop >= d.base;
+    const u64* __restrict__ mtab = fp ? reinterpret_cast<const u64*>(sprcp) : spinv;
+    auto place = [&](u32 q, u64 m, u64& o) {
+      return fp ? first_offset_fp(bd, d.base, q, __builtin_bit_cast(double, m), span, chk_sq, o)
+                : first_offset(d.base, q, m, span, o);
+    };
+    // tier A: one wavefront per prime, lanes stride over the marks of each class.  A lane's
+    // step, 64 q bytes, is a multiple of 4: its marks of one class keep their byte within the
+    // word, so the bit is set up once and the loop advances a word index.
     const u32 na = min(tierA, nsmall);
     u32 qa = PRE_COUNT + wave < na ? sp[PRE_COUNT + wave] : 0;
-    u64 ma = PRE_COUNT + wave < na ? spinv[PRE_COUNT + wave] : 0;
+    u64 ma = PRE_COUNT + wave < na ? mtab[PRE_COUNT + wave] : 0;
     for (u32 i = PRE_COUNT + wave; i < na; i += nwaves) {
       const u32 q = qa;
-      const u64 minv = ma;
+      const u64 m = ma;
       if (i + nwaves < na) {      // the wavefront's next prime, loaded while this one is sieved
         qa = sp[i + nwaves];
-        ma = spinv[i + nwaves];
+        ma = mtab[i + nwaves];
       }
       u64 o;
-      if (!first_offset(d.base, q, minv, span, o)) continue;
+      if (!place(q, m, o)) continue;
       const int qi = wl.qi[q % 30];
       const u32 jl = (u32)class_start(o, q, class_ms(o, qi, wl), lane & 7);   // lane c (< 8): class c's start
       for (int c = 0; c < 8; c++) {
-        const u32 j0 = __shfl(jl, c, 64);
-        for (u32 j = j0 + (u32)lane * q; j < d.nbytes; j += 64u * q)
-          atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c));
+        const u32 j = __shfl(jl, c, 64) + (u32)lane * q;   // q < 512: starts lie within the span
+        if (j >= d.nbytes) continue;
+        const u32 bit = 1u << (((j & 3) << 3) | c);
+        for (u32 w = j >> 2; w < nwords; w += 16u * q) atomicOr(&seg[w], bit);
       }
     }
     // tier B: one lane per prime; the next prime's q and Barrett constant are loaded
@@ -471,16 +518,16 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
     {
       u32 i = tierA + threadIdx.x;
       u32 qn = i < nsmall ? sp[i] : 0;
-      u64 mn = i < nsmall ? spinv[i] : 0;
+      u64 mn = i < nsmall ? mtab[i] : 0;
       for (; i < nsmall; i += blockDim.x) {
         const u32 q = qn;
-        const u64 minv = mn;
+        const u64 m = mn;
         if (i + blockDim.x < nsmall) {
           qn = sp[i + blockDim.x];
-          mn = spinv[i + blockDim.x];
+          mn = mtab[i + blockDim.x];
         }
         u64 o;
-        if (!first_offset(d.base, q, minv, span, o)) continue;
+        if (!place(q, m, o)) continue;
         const int qi = wl.qi[q % 30];
         if ((u64)WHEEL_DIV * q > d.nbytes) {     // fewer than WHEEL_DIV marks per class
           auto mark = [&](u32 j, int c) { atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c)); };
@@ -1365,6 +1412,7 @@ struct wf_engine {
   u64 sp_lim = 0;
   u32* d_sp = nullptr;
   u64* d_spinv = nullptr;
+  double* d_sprcp = nullptr;     // 1/q rounded to nearest (placement below 2^53)
   u32 tierA = 0, q1idx = 0;
   double surv_frac = 1.0;        // share of numbers with no prime factor below bucket_min
   size_t fill_hint = 0;          // fill items of the last planned pass (a reserve size)
@@ -1495,13 +1543,21 @@ static int ensure_sieving_primes(wf_engine* e, u64 lim) {
   for (size_t i = 0; i < e->h_sp.size(); i++) inv[i] = 0xFFFFFFFFFFFFFFFFull / e->h_sp[i];
   if (e->d_sp) (void)hipFree(e->d_sp);
   if (e->d_spinv) (void)hipFree(e->d_spinv);
+  if (e->d_sprcp) (void)hipFree(e->d_sprcp);
+  e->d_sprcp = nullptr;
   e->d_sp = nullptr;
   e->d_spinv = nullptr;
   const size_t n = std::max<size_t>(e->h_sp.size(), 1);
   if (hipMalloc(&e->d_sp, n * sizeof(u32)) != hipSuccess) return WF_E_NOMEM;
   if (hipMalloc(&e->d_spinv, n * sizeof(u64)) != hipSuccess) return WF_E_NOMEM;
+  if (hipMalloc(&e->d_sprcp, n * sizeof(double)) != hipSuccess) return WF_E_NOMEM;
   HIPCHK(hipMemcpy(e->d_sp, e->h_sp.data(), e->h_sp.size() * sizeof(u32), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(e->d_spinv, inv.data(), inv.size() * sizeof(u64), hipMemcpyHostToDevice));
+  {
+    std::vector<double> rcp(e->h_sp.size());
+    for (size_t i = 0; i < e->h_sp.size(); i++) rcp[i] = 1.0 / (double)e->h_sp[i];
+    HIPCHK(hipMemcpy(e->d_sprcp, rcp.data(), rcp.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   e->sp_lim = want;
   // tier A: primes below 64 * 8 = 512 are striped over a wavefront
   e->tierA = (u32)(std::lower_bound(e->h_sp.begin(), e->h_sp.end(), 512u) - e->h_sp.begin());
@@ -1659,6 +1715,7 @@ extern "C" void wf_close(wf_engine* e) {
   if (e->st) (void)hipStreamSynchronize(e->st);
   (void)hipFree(e->d_sp);
   (void)hipFree(e->d_spinv);
+  (void)hipFree(e->d_sprcp);
   (void)hipFree(e->d_segs);
   (void)hipFree(e->d_runs);
   (void)hipFree(e->d_fills);
@@ -2054,7 +2111,8 @@ static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, 
   }
   const u32 sgrid = (u32)std::min<u64>(nseg, (u64)cus * 4);
   hipLaunchKernelGGL(k_sieve, dim3(sgrid), dim3(e->cfg.sieve_block), e->cfg.seg_bytes, st, dsegs, (u32)nseg, e->d_sp,
-                     e->d_spinv, e->tierA, e->q1idx, B.bcnt, B.bdata, P.bcap, B.off, B.cnt, P.pcap, acc, e->d_flags,
+                     e->d_spinv, e->d_sprcp, e->tierA, e->q1idx, B.bcnt, B.bdata, P.bcap, B.off, B.cnt, P.pcap, acc,
+                     e->d_flags,
                      want_bitmap ? B.bitmap : nullptr, e->cfg.seg_bytes / 4, e->d_pre, e->pre_words);
   HIPCHK(hipGetLastError());
   if (P.nprove) {
