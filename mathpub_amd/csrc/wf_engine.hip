@@ -536,6 +536,23 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
           continue;
         }
         const u64 ms = class_ms(o, qi, wl);
+        if (8u * q <= d.nbytes) {
+          // eight or more marks per class: the marks j0 + (4k + t) q of one t keep their byte
+          // within the word (4 q bytes = q words), so each of the four runs sets one bit
+          // through a word index
+#pragma unroll 1
+          for (int c = 0; c < 8; c++) {
+            const u32 j0 = (u32)class_start(o, q, ms, c);
+            if (j0 >= d.nbytes) continue;
+#pragma unroll
+            for (u32 t = 0; t < 4; t++) {
+              const u32 j = j0 + t * q;
+              const u32 bit = 1u << (((j & 3) << 3) | c);
+              for (u32 w = j >> 2; w < nwords; w += q) atomicOr(&seg[w], bit);
+            }
+          }
+          continue;
+        }
 #pragma unroll 1
         for (int c = 0; c < 8; c++) {
           for (u32 j = (u32)class_start(o, q, ms, c); j < d.nbytes; j += q)
