@@ -1032,6 +1032,23 @@ __device__ __forceinline__ u32 mod_barrett(u64 x, u32 q, u64 minv) {
   return (u32)r;
 }
 
+__host__ __device__ __forceinline__ u32 umin32(u32 x, u32 y) { return x < y ? x : y; }
+
+// (-base) mod q for 512 <= q < 2^31 and a base below 2^53, from rq = 1/q rounded to nearest,
+// with one-sided error.  base / q < 2^44, so base * rq is within 2^-9 of base / q (rq's
+// relative error is at most 2^-53) and the FMA that subtracts 2^-7 rounds by at most 2^-9 more:
+// its result lies in [base/q - 2^-7 - 2^-8, base/q - 2^-7 + 2^-8], below base / q and above
+// base / q - 1.  So t = floor of it is floor(base / q) or one less, and r = base - t q (exact in
+// the FMA, an integer in [0, 2q)) needs one correction; unsigned min(x, x - q) is x mod q for
+// x < 2q.  Ten VALU instructions against twenty for the two-sided correction in doubles.
+__host__ __device__ __forceinline__ u32 neg_mod_fp(double bd, u32 q, double rq) {
+  const double t = __builtin_floor(__builtin_fma(bd, rq, -0.0078125));
+  const u32 r2 = (u32)__builtin_fma(-t, (double)q, bd);
+  const u32 r = umin32(r2, r2 - q);
+  const u32 o = q - r;        // (0, q]
+  return umin32(o, o - q);
+}
+
 // One prime of the order sieve.  A base-2 strong pseudoprime n (indeed any Fermat one) has
 // 2^(n-1) = 1 (mod p) for each prime factor p, so ord_p(2) | n - 1.  With lcm(2, ord_p(2)) =
 // s2, an odd multiple n = m p can be one only if m = u (mod s2), u = p^-1 mod s2 (p m = 1

@@ -187,7 +187,15 @@ __device__ __forceinline__ void wheel_walk(T o, u32 q, int qi, T span, const Whe
 #ifndef FILL_STAGE
 #define FILL_STAGE 16384  // records a fill workgroup stages in LDS (64 KiB)
 #endif
+#ifndef FILL_TARGET
 #define FILL_TARGET (FILL_STAGE / 2)  // expected records per fill work item (the host sizes items to it)
+#endif
+#ifndef FILL_PER_PRIME
+#define FILL_PER_PRIME 64 // records per prime and fill group the host sizes groups for
+#endif
+#ifndef FILL_PAIRS_DIV
+#define FILL_PAIRS_DIV 1  // an item takes one lane per (prime, class) below blockDim / FILL_PAIRS_DIV primes
+#endif
 // Wave priority of the sieve-stage kernels.  In the pipelined path they share the CUs
 // with the previous pass's test kernel, which alone keeps the VALU nearly saturated; a
 // higher priority lets their (latency-bound) waves issue first, and the tests take the
@@ -229,7 +237,7 @@ struct FillRun { u64 base; u32 seg0, nseg, lseg0, pad; };
 
 // one k_bucket_fill work item: sieving primes [i0, i1) over the fill runs fr0 .. fr0 + nfr - 1,
 // nseg segments in all (<= RUN_MAX)
-struct FillDesc { u32 fr0, nfr, i0, i1, nseg, pad; };
+struct FillDesc { u32 fr0, nfr, i0, i1, nseg, fp; };   // fp: every run lies below 2^53
 
 struct DevAcc {     // per range; every field is a 64-bit sum (checksums are per range and base)
   unsigned long long primes, tests, near, flt, nev, npsp, nbpsw;
@@ -296,12 +304,30 @@ __device__ __forceinline__ bool first_offset_fp(double bd, u64 base, u32 q, doub
 template <class F, class F4>
 __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __restrict__ fruns,
                                           const u32* __restrict__ sp, const u64* __restrict__ spinv,
-                                          const WheelLds& wl, int sbshift, bool pairs, F hit, F4 hit4) {
+                                          const double* __restrict__ sprcp, const WheelLds& wl, int sbshift,
+                                          bool pairs, F hit, F4 hit4) {
   const u32 nwork = pairs ? (it.i1 - it.i0) * 8 : it.i1 - it.i0;
-  for (u32 w = threadIdx.x; w < nwork; w += blockDim.x) {
+  // Placement: for an item below 2^53 (its primes are below 2^26.5) by neg_mod_fp from the
+  // FP64 reciprocals, else by Barrett reduction.  The next work unit's prime and constant are
+  // loaded while this one is walked.
+  const bool fpf = it.fp != 0;
+  const u64* __restrict__ mtab = fpf ? reinterpret_cast<const u64*>(sprcp) : spinv;
+  u32 w = threadIdx.x;
+  u32 qn = 0;
+  u64 mn = 0;
+  if (w < nwork) {
     const u32 i = it.i0 + (pairs ? w >> 3 : w);
-    const u32 q = sp[i];
-    const u64 minv = spinv[i];
+    qn = sp[i];
+    mn = mtab[i];
+  }
+  for (; w < nwork; w += blockDim.x) {
+    const u32 q = qn;
+    const u64 minv = mn;
+    if (w + blockDim.x < nwork) {
+      const u32 i = it.i0 + (pairs ? (w + blockDim.x) >> 3 : w + blockDim.x);
+      qn = sp[i];
+      mn = mtab[i];
+    }
     const int qi = wl.qi[q % 30];
     const int c0 = pairs ? (int)(w & 7) : 0, c1 = pairs ? c0 + 1 : 8;
     for (u32 r = 0; r < it.nfr; r++) {
@@ -309,7 +335,13 @@ __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __r
       const u64 nbytes = (u64)fr.nseg << sbshift;
       const u64 span = 30ull * nbytes;
       u64 o;
-      if (!first_offset(fr.base, q, minv, span, o)) continue;
+      if (fpf) {
+        const u64 qq = (u64)q * q;
+        o = qq >= fr.base ? qq - fr.base : neg_mod_fp((double)fr.base, q, __builtin_bit_cast(double, minv));
+        if (o >= span) continue;
+      } else if (!first_offset(fr.base, q, minv, span, o)) {
+        continue;
+      }
       if (!pairs && (u64)WALK_DIV * q > span) {
         wheel_walk<u64>(o, q, qi, span, wl, [&](u32 j, int c) { hit(fr.lseg0 + (j >> sbshift), j, (u32)c); });
         continue;
@@ -346,7 +378,7 @@ __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __r
 __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __restrict__ items,
                                                      const FillRun* __restrict__ fruns,
                                                      const u32* __restrict__ sp, const u64* __restrict__ spinv,
-                                                     u32* __restrict__ bcnt, u32* __restrict__ bdata,
+                                                     const double* __restrict__ sprcp, u32* __restrict__ bcnt, u32* __restrict__ bdata,
                                                      u32 bcap, int sbshift, u32* __restrict__ flags) {
   __shared__ u32 cur[RUN_MAX];    // records per segment (stage slots taken), then the count to copy
   __shared__ union {
@@ -379,8 +411,8 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
     }
   };
   // few primes with many hits each: one lane per (prime, class); else one lane per prime
-  const bool pairs = it.i1 - it.i0 < blockDim.x;
-  fill_walk(it, fruns, sp, spinv, wl, sbshift, pairs, [&](u32 sg, u32 j, u32 c) {
+  const bool pairs = (it.i1 - it.i0) * FILL_PAIRS_DIV < blockDim.x;
+  fill_walk(it, fruns, sp, spinv, sprcp, wl, sbshift, pairs, [&](u32 sg, u32 j, u32 c) {
     place(sg, atomicAdd(&cur[sg], 1u), ((j & smask) << 3) | c);
   }, [&](const u32 (&sg)[4], const u32 (&jj)[4], u32 c) {
     u32 slot[4];
@@ -415,10 +447,22 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
     cur[k] = n;
   }
   __syncthreads();
-  const u32 nslot = it.nseg << cshift;
-  for (u32 x = threadIdx.x; x < nslot; x += blockDim.x) {
-    const u32 s = x >> cshift, i = x & (C - 1);
-    if (i < cur[s]) bdata[sw.dst[s] + i] = stage[x];
+  if (C >= 128) {
+    // 64 or more expected records per segment: a wavefront per segment copies its filled
+    // slots only, consecutive lanes on consecutive addresses
+    const u32 lane = threadIdx.x & 63;
+    for (u32 s = threadIdx.x >> 6; s < it.nseg; s += blockDim.x >> 6) {
+      const u32 n = cur[s];
+      const u32* src = stage + (s << cshift);
+      u32* dst = bdata + sw.dst[s];
+      for (u32 i = lane; i < n; i += 64) dst[i] = src[i];
+    }
+  } else {
+    // fewer: the threads cover the slots (C <= 64 divides the block), each keeping its slot
+    // index i while it steps over the segments
+    const u32 i = threadIdx.x & (C - 1), step = blockDim.x >> cshift;
+    for (u32 s = threadIdx.x >> cshift; s < it.nseg; s += step)
+      if (i < cur[s]) bdata[sw.dst[s] + i] = stage[(s << cshift) + i];
   }
 }
 
@@ -489,6 +533,20 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
       return fp ? first_offset_fp(bd, d.base, q, __builtin_bit_cast(double, m), span, chk_sq, o)
                 : first_offset(d.base, q, m, span, o);
     };
+    // tier B (q >= 512): below 2^53 the one-sided FP64 placement (neg_mod_fp, q < 2^31)
+    const bool fpb = fp && qtop < 0x80000000ull;
+    auto place_b = [&](u32 q, u64 m, u64& o) {
+      if (!fpb) return place(q, m, o);
+      if (chk_sq) {
+        const u64 qq = (u64)q * q;
+        if (qq >= d.base) {
+          o = qq - d.base;
+          return o < span;
+        }
+      }
+      o = neg_mod_fp(bd, q, __builtin_bit_cast(double, m));
+      return o < span;
+    };
     // tier A: one wavefront per prime, lanes stride over the marks of each class.  A lane's
     // step, 64 q bytes, is a multiple of 4: its marks of one class keep their byte within the
     // word, so the bit is set up once and the loop advances a word index.
@@ -527,7 +585,7 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
           mn = mtab[i + blockDim.x];
         }
         u64 o;
-        if (!place(q, m, o)) continue;
+        if (!place_b(q, m, o)) continue;
         const int qi = wl.qi[q % 30];
         if ((u64)WHEEL_DIV * q > d.nbytes) {     // fewer than WHEEL_DIV marks per class
           auto mark = [&](u32 j, int c) { atomicOr(&seg[j >> 2], 1u << (((j & 3) << 3) | c)); };
@@ -2050,8 +2108,9 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
       const u32 qend = Q >= 0x80000000u ? 0xFFFFFFFFu : 2 * Q;
       const u32 gend = (u32)(std::lower_bound(e->h_sp.begin() + i, e->h_sp.end(), qend) - e->h_sp.begin());
       u32 G = 1;
-      while (G < RUN_MAX && 2.0 * G * 8.0 * SBd <= 64.0 * Q) G *= 2;
+      while (G < RUN_MAX && 2.0 * G * 8.0 * SBd <= FILL_PER_PRIME * (double)Q) G *= 2;
       u32 g_fr0 = (u32)P.fruns.size(), g_nseg = 0, g_nq = 0;
+      u64 g_top = 0;   // end of the group's highest run
       auto flush = [&]() {
         if (!g_nseg) return;
         const u32 bend = std::min(gend, g_nq);
@@ -2065,13 +2124,14 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
           f.i0 = a;
           f.i1 = b;
           f.nseg = g_nseg;
-          f.pad = 0;
+          f.fp = g_top <= (1ull << 53) ? 1u : 0u;
           P.fills.push_back(f);
           a = b;
         }
         g_fr0 = (u32)P.fruns.size();
         g_nseg = 0;
         g_nq = 0;
+        g_top = 0;
       };
       for (const RunDesc& ch : P.runs) {
         if (ch.nq <= i) continue;
@@ -2087,6 +2147,8 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
           P.fruns.push_back(fr);
           g_nseg += ns;
           g_nq = std::max(g_nq, ch.nq);
+          // (a base at or above 2^53 counts as the top: base + span could wrap near 2^64)
+          g_top = std::max<u64>(g_top, fr.base >= (1ull << 53) ? ~0ull : fr.base + (u64)ns * 30ull * e->cfg.seg_bytes);
         }
       }
       flush();
@@ -2122,7 +2184,7 @@ static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, 
     hipLaunchKernelGGL(k_zero, dim3((u32)std::min<u64>((nseg + 255) / 256, 1024)), dim3(256), 0, st, B.bcnt, nseg);
     if (!P.fills.empty()) {
       hipLaunchKernelGGL(k_bucket_fill, dim3((u32)P.fills.size()), dim3(FILL_BLOCK), 0, st, dfills, druns, e->d_sp,
-                         e->d_spinv, B.bcnt, B.bdata, P.bcap, e->sbshift, e->d_flags);
+                         e->d_spinv, e->d_sprcp, B.bcnt, B.bdata, P.bcap, e->sbshift, e->d_flags);
       HIPCHK(hipGetLastError());
     }
   }

@@ -180,4 +180,37 @@ void emu_fp64c2(uint32_t baseA, uint32_t baseB, uint64_t p, int ebits, uint64_t*
   finish(p, a1, b1, r + 4);
 }
 
+// (-base) mod q by the sieve's one-sided FP64 placement (neg_mod_fp, rq = 1/q rounded to nearest)
+uint32_t emu_neg_mod_fp(uint64_t base, uint32_t q) { return neg_mod_fp((double)base, q, 1.0 / (double)q); }
+
+// n random (base < 2^53, 512 <= q < 2^31) pairs from a splitmix64 stream, half of them with
+// base / q within 2^-6 of an integer on either side; returns how many differ from the exact
+// (-base) mod q
+uint64_t emu_neg_mod_fp_random(uint64_t n, uint64_t seed) {
+  uint64_t bad = 0, z = seed;
+  auto next = [&]() {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  for (uint64_t k = 0; k < n; k++) {
+    const int qb = 10 + (int)(next() % 22);                    // q of 10 to 31 bits
+    uint32_t q = (uint32_t)(next() >> (64 - qb)) | (1u << (qb - 1));
+    if (q < 512) q = 512;
+    if (q >= 0x80000000u) q = 0x7FFFFFFFu;
+    uint64_t base = next() >> 11;                               // below 2^53
+    if (k & 1) {                                                // base / q near an integer
+      const uint64_t m = base / q;
+      const uint64_t d = next() % (q / 64 + 1);
+      base = (next() & 2) ? m * q + d : (m ? m * q - 1 - d : d);
+      if (base >= (1ull << 53)) base = (1ull << 53) - 1 - d;
+    }
+    const uint32_t want = (uint32_t)((q - base % q) % q);
+    if (emu_neg_mod_fp(base, q) != want) bad++;
+  }
+  return bad;
+}
+
 }  // extern "C"

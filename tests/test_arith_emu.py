@@ -32,6 +32,10 @@ def emu():
     L.emu_m28c.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, P, P]
     L.emu_psp_order.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8)]
     L.emu_fp64c2.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, P, ctypes.c_int]
+    L.emu_neg_mod_fp.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
+    L.emu_neg_mod_fp.restype = ctypes.c_uint32
+    L.emu_neg_mod_fp_random.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.emu_neg_mod_fp_random.restype = ctypes.c_uint64
     return L
 
 
@@ -507,3 +511,27 @@ def test_fp64_two_bases_one_prime(emu):
                     if out[4 * k + 2]:
                         assert out[4 * k + 3] == (r - 1) // p, (b, p)
 
+
+
+def test_neg_mod_fp_edges(emu):
+    """The sieve's and fill's one-sided FP64 placement: (-base) mod q for 512 <= q < 2^31 and
+    base < 2^53.  The quotient estimate is biased low, so the cases that would break a wrong
+    bias are base / q at or just around an integer, at both ends of the base range and for the
+    smallest and largest q."""
+    top = (1 << 53) - 1
+    qs = [512, 521, 1021, 65521, 1048573, (1 << 24) - 3, 94906249, (1 << 31) - 1]
+    for q in qs:
+        for m in (0, 1, 2, 12345, top // q - 1, top // q):
+            for d in (0, 1, 2, q // 256, q // 128, q // 128 + 1, q // 64, q // 2, q - 2, q - 1):
+                base = m * q + d
+                if base > top:
+                    continue
+                assert emu.emu_neg_mod_fp(base, q) == (-base) % q, (base, q)
+        for base in (0, 1, q - 1, q, q + 1, top, top - 1, top - q, 10 ** 15, 10 ** 15 + 10 ** 11):
+            assert emu.emu_neg_mod_fp(base, q) == (-base) % q, (base, q)
+
+
+def test_neg_mod_fp_random(emu):
+    """Four million random (base, q) pairs, half of them with base / q within 2^-6 of an
+    integer, against exact integer arithmetic (inside the harness)."""
+    assert emu.emu_neg_mod_fp_random(4_000_000, 0x243F6A8885A308D3) == 0
