@@ -27,7 +27,7 @@ FP64_PEAK_INSTR = 256 * 64 * 2.4e9   # wave64 FP64 VALU instructions: lanes per 
 # 32x32->64 products per second of v_mad_u64_u32 over the whole chip, measured by
 # mathpub_amd/wf_probe valu (profiles/valu_rates.txt: 35.47 lane-ops per clock and CU)
 INT32_MUL_PEAK = 2.179e13
-K_TEST_VALU_PER_TEST = 795           # k_test<A_F64> at [1e15, 1e15+1e11): profiles/c2_sq_counters_pmc.md
+K_TEST_VALU_PER_TEST = 784           # k_test<A_F64> at [1e15, 1e15+1e11): profiles/c2_sq_counters_pmc.md
 
 
 def wf_products(pbits):

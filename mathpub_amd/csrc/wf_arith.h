@@ -63,7 +63,9 @@ __device__ __forceinline__ u64 inv64(u64 p) {
 //   b'  = k1 + 2r        |b'| <= alpha^2 p + 1 + (1 + 0.5 alpha beta) p, so
 //                       beta' = 1.34 + 0.29 beta (fixed point 1.9).
 // All quotients stay below 2^51 in magnitude (round_mul's range) and every remainder below
-// 2^52, so each FMA result is exact.
+// 2^52, so each FMA result is exact.  ip may be 1/p correctly rounded or within
+// 2^-53 (1 + 2^-7) of it (f64_recip_near): the 0.25 alpha^2 terms grow by 2^-8 of themselves,
+// and the fixed points above keep their digits.
 struct F64Mod {
   double p, ip;   // p, 1/p
 };
@@ -155,14 +157,16 @@ __device__ __forceinline__ void fp64_pow2(u64 p, u64 e, int ebits, double& ra, d
 // k = round(2^t / p) < 2^31 < p/2 and a = 2^t - k p exact, |a| <= p/2; then a first window
 // of (ebits - 6) mod W bits, so that the W-bit windows end at bit 0.  Against the plain
 // leading digit this saves 6 - (ebits mod W or W) squarings: one at 50 bits.
+//
+// ip0, ip1: 1/p within 2^-53 (1 + 2^-7) of it, relative (1.0 / pd, or f64_recip_near).
 template <int W, bool BIG = false>
-__device__ __forceinline__ void fp64_pow2x2(double pd0, u64 e0, double pd1, u64 e1, int ebits, double& a0,
-                                            double& b0, double& a1, double& b1) {
+__device__ __forceinline__ void fp64_pow2x2_r(double pd0, double ip0, u64 e0, double pd1, double ip1, u64 e1,
+                                              int ebits, double& a0, double& b0, double& a1, double& b1) {
   F64Mod m0, m1;
   m0.p = pd0;
-  m0.ip = 1.0 / pd0;
+  m0.ip = ip0;
   m1.p = pd1;
-  m1.ip = 1.0 / pd1;
+  m1.ip = ip1;
   int sh;
   if (BIG) {
     sh = ebits - 6;
@@ -205,6 +209,24 @@ __device__ __forceinline__ void fp64_pow2x2(double pd0, u64 e0, double pd1, u64 
       f64_mul_pow2(a1, b1, (int)((e1 >> sh) & ((1u << W) - 1)), m1);
     }
   }
+}
+
+template <int W, bool BIG = false>
+__device__ __forceinline__ void fp64_pow2x2(double pd0, u64 e0, double pd1, u64 e1, int ebits, double& a0,
+                                            double& b0, double& a1, double& b1) {
+  fp64_pow2x2_r<W, BIG>(pd0, 1.0 / pd0, e0, pd1, 1.0 / pd1, e1, ebits, a0, b0, a1, b1);
+}
+
+// 1/p for p = base + f, base >= 2^40 and 0 <= f < 2^22 (pd = p, fd = f, exact as doubles), from
+// ib = 1/base rounded to nearest.  The first-order estimate y = ib (1 - f ib) has a relative
+// error below (f/base)^2 + 3 * 2^-53 < 2^-35, and one Newton step y + y (1 - p y) squares it:
+// the result is within 2^-53 + 2^-70 of 1/p, relative (the inner FMA's own rounding adds
+// 2^-35 * 2^-53).  Four FP64 instructions per prime against about twelve for the correctly
+// rounded division; the squaring's bounds take 1/p with a relative error up to 2^-53 (1 + 2^-7).
+__host__ __device__ __forceinline__ double f64_recip_near(double ib, double fd, double pd) {
+  const double y = ib * __builtin_fma(-fd, ib, 1.0);
+  const double e = __builtin_fma(-pd, y, 1.0);
+  return __builtin_fma(y, e, y);
 }
 
 // The digits of r = a + b p as a ladder leaves them: |a| <= p/2 (1 + 2^-20) after a window
@@ -259,13 +281,14 @@ __device__ __forceinline__ void f64_mul_small_big(double& a, double& b, double c
 // on one prime).  x = b^(leading digit), then per window W squarings and one multiplication
 // by b^d (BIG: both p >= 2^32).  The digits are left raw, as fp64_pow2x2 leaves them.
 template <int W, bool BIG = false>
-__device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64 e1, int ebits, const u32* pw0,
-                                             const u32* pw1, double& a0, double& b0, double& a1, double& b1) {
+__device__ __forceinline__ void fp64_powc_x2_r(double pd0, double ip0, u64 e0, double pd1, double ip1, u64 e1,
+                                               int ebits, const u32* pw0, const u32* pw1, double& a0, double& b0,
+                                               double& a1, double& b1) {
   F64Mod m0, m1;
   m0.p = pd0;
-  m0.ip = 1.0 / pd0;
+  m0.ip = ip0;
   m1.p = pd1;
-  m1.ip = 1.0 / pd1;
+  m1.ip = ip1;
   const int nw = (ebits + W - 1) / W;
   int sh = (nw - 1) * W;
   const u32 c0 = pw0[(int)(e0 >> sh) & ((1 << W) - 1)], c1 = pw1[(int)(e1 >> sh) & ((1 << W) - 1)];
@@ -288,6 +311,12 @@ __device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64
       f64_mul_small(a1, b1, (double)pw1[(int)((e1 >> sh) & ((1u << W) - 1))], m1);
     }
   }
+}
+
+template <int W, bool BIG = false>
+__device__ __forceinline__ void fp64_powc_x2(double pd0, u64 e0, double pd1, u64 e1, int ebits, const u32* pw0,
+                                             const u32* pw1, double& a0, double& b0, double& a1, double& b1) {
+  fp64_powc_x2_r<W, BIG>(pd0, 1.0 / pd0, e0, pd1, 1.0 / pd1, e1, ebits, pw0, pw1, a0, b0, a1, b1);
 }
 
 // ---------------------------------------------------------------------------------

@@ -1005,9 +1005,6 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
   __syncthreads();
   LaneSums s = {0, 0, 0, 0, 0};
   u32 cur = 0xFFFFFFFFu, curj = 0;
-  // a valid modulus of the class for idle lanes
-  const u64 idle = ARITH == A_F64 ? 3ull : ARITH == A_M2 ? 3ull : ARITH == A_M3 ? 2147483659ull
-                 : ARITH == A_M4 ? 140737488355333ull : 0xFFFFFFFFFFFFFFC5ull;   // FULL / M28
   const u64 nitems = (u64)nseg * tps * o.nb;
   for (u64 t = blockIdx.x; t < nitems; t += gridDim.x) {
     const u32 j = (u32)(t % o.nb);
@@ -1034,38 +1031,51 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
     const int eb = etop ? 64 - __builtin_clzll(etop) : 1;
     if (ARITH == A_F64) {
       // two primes per lane: i and i + blockDim.x.  A segment at or above 2^32 uses the
-      // shorter window step (idle lanes then take the prime 2^32 + 15).
+      // shorter window step; at or above 2^40, 1/p comes from the segment's 1/base
+      // (f64_recip_near).  A lane past the tile's end re-tests the tile's last prime, and its
+      // result is not accounted.
       const bool big = d.base >= (1ull << 32);
-      const u64 idle_f = big ? 4294967311ull : idle;
+      const bool near = d.base >= (1ull << 40);
       const double based = (double)d.base;     // p = base + offset < 2^50: exact as a double
+      const double ib = near ? 1.0 / based : 0.0;
       for (u32 base = beg; base < end; base += 2 * blockDim.x) {
         const u32 i0 = base + threadIdx.x, i1 = i0 + blockDim.x;
         const bool act0 = i0 < end, act1 = i1 < end;
-        const u32 f0 = act0 ? po[i0] : 0, f1 = act1 ? po[i1] : 0;
-        const u64 p0 = act0 ? d.base + f0 : idle_f, p1 = act1 ? d.base + f1 : idle_f;
-        const double pd0 = act0 ? based + (double)f0 : (double)idle_f;
-        const double pd1 = act1 ? based + (double)f1 : (double)idle_f;
+        const u32 f0 = po[min(i0, end - 1)], f1 = po[min(i1, end - 1)];
+        const u64 p0 = d.base + f0, p1 = d.base + f1;
+        const double fd0 = (double)f0, fd1 = (double)f1;
+        const double pd0 = based + fd0, pd1 = based + fd1;
+        double ip0, ip1;
+        if (near) {
+          ip0 = f64_recip_near(ib, fd0, pd0);
+          ip1 = f64_recip_near(ib, fd1, pd1);
+        } else {
+          ip0 = 1.0 / pd0;
+          ip1 = 1.0 / pd1;
+        }
         double a0, b0, a1, b1;
         if (b == 2 && big) {
-          fp64_pow2x2<5, true>(pd0, p0 - 1, pd1, p1 - 1, eb, a0, b0, a1, b1);
+          fp64_pow2x2_r<5, true>(pd0, ip0, p0 - 1, pd1, ip1, p1 - 1, eb, a0, b0, a1, b1);
         } else if (b == 2) {
-          fp64_pow2x2<5>(pd0, p0 - 1, pd1, p1 - 1, eb, a0, b0, a1, b1);
+          fp64_pow2x2_r<5>(pd0, ip0, p0 - 1, pd1, ip1, p1 - 1, eb, a0, b0, a1, b1);
         } else if (big) {
+#define WF_POWC(W_, BIG_) fp64_powc_x2_r<W_, BIG_>(pd0, ip0, p0 - 1, pd1, ip1, p1 - 1, eb, pw, pw, a0, b0, a1, b1)
           switch (w) {
-            case 5: fp64_powc_x2<5, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
-            case 4: fp64_powc_x2<4, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
-            case 3: fp64_powc_x2<3, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
-            case 2: fp64_powc_x2<2, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
-            default: fp64_powc_x2<1, true>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            case 5: WF_POWC(5, true); break;
+            case 4: WF_POWC(4, true); break;
+            case 3: WF_POWC(3, true); break;
+            case 2: WF_POWC(2, true); break;
+            default: WF_POWC(1, true); break;
           }
         } else {
           switch (w) {
-            case 5: fp64_powc_x2<5>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
-            case 4: fp64_powc_x2<4>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
-            case 3: fp64_powc_x2<3>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
-            case 2: fp64_powc_x2<2>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
-            default: fp64_powc_x2<1>(pd0, p0 - 1, pd1, p1 - 1, eb, pw, pw, a0, b0, a1, b1); break;
+            case 5: WF_POWC(5, false); break;
+            case 4: WF_POWC(4, false); break;
+            case 3: WF_POWC(3, false); break;
+            case 2: WF_POWC(2, false); break;
+            default: WF_POWC(1, false); break;
           }
+#undef WF_POWC
         }
         if (act0 && !divides_base(p0, b)) account_padic(o, d.rid, j, p0, pd0, a0, b0, s, nT);
         if (act1 && !divides_base(p1, b)) account_padic(o, d.rid, j, p1, pd1, a1, b1, s, nT);
@@ -1074,7 +1084,7 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
       for (u32 base = beg; base < end; base += blockDim.x) {
         const u32 i = base + threadIdx.x;
         const bool active = i < end;
-        const u64 p = active ? d.base + po[i] : idle;
+        const u64 p = d.base + po[min(i, end - 1)];   // an idle lane re-tests the tile's last prime
         u64 rhi, rlo;
         residue<ARITH>(p, eb, b, w, pw, rhi, rlo);
         if (active && !divides_base(p, b)) account(o, d.rid, j, p, rhi, rlo, s, nT);

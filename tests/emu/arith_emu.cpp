@@ -213,4 +213,22 @@ uint64_t emu_neg_mod_fp_random(uint64_t n, uint64_t seed) {
   return bad;
 }
 
+// 1/p by f64_recip_near from the base's reciprocal (p = base + f)
+double emu_recip_near(uint64_t base, uint32_t f) {
+  const double pd = (double)(base + f);
+  return f64_recip_near(1.0 / (double)base, (double)f, pd);
+}
+
+// the kernel's two ladders above 2^40: p_k = base + f_k, 1/p_k from f64_recip_near
+void emu_fp64x2_near(uint64_t base, uint32_t f0, uint32_t f1, int ebits, uint64_t* r) {
+  const double ib = 1.0 / (double)base;
+  const uint64_t p0 = base + f0, p1 = base + f1;
+  const double pd0 = (double)p0, pd1 = (double)p1;
+  double a0, b0, a1, b1;
+  fp64_pow2x2_r<5, true>(pd0, f64_recip_near(ib, (double)f0, pd0), p0 - 1, pd1, f64_recip_near(ib, (double)f1, pd1),
+                         p1 - 1, ebits, a0, b0, a1, b1);
+  finish(p0, a0, b0, r);
+  finish(p1, a1, b1, r + 4);
+}
+
 }  // extern "C"

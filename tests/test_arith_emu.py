@@ -36,6 +36,9 @@ def emu():
     L.emu_neg_mod_fp.restype = ctypes.c_uint32
     L.emu_neg_mod_fp_random.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     L.emu_neg_mod_fp_random.restype = ctypes.c_uint64
+    L.emu_recip_near.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
+    L.emu_recip_near.restype = ctypes.c_double
+    L.emu_fp64x2_near.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, P]
     return L
 
 
@@ -535,3 +538,40 @@ def test_neg_mod_fp_random(emu):
     """Four million random (base, q) pairs, half of them with base / q within 2^-6 of an
     integer, against exact integer arithmetic (inside the harness)."""
     assert emu.emu_neg_mod_fp_random(4_000_000, 0x243F6A8885A308D3) == 0
+
+
+def test_recip_near_bound(emu):
+    """f64_recip_near: 1/p from the segment's 1/base (base >= 2^40, offset f < 2^22) within
+    2^-53 (1 + 2^-7) of 1/p, relative, checked exactly with rationals; the edges are the
+    smallest base, the largest offset and p just below 2^50."""
+    from fractions import Fraction
+    rng = random.Random(7)
+    cases = [((1 << 40), (1 << 22) - 1), ((1 << 40), 1), ((1 << 50) - (1 << 22), (1 << 22) - 1),
+             (10 ** 15, 0), (10 ** 15, 1 << 21), (30 * ((1 << 45) // 30), 12345)]
+    for _ in range(20000):
+        base = 30 * (rng.randrange(1 << 40, (1 << 50) - (1 << 22)) // 30)
+        if base < 1 << 40:
+            base += 30
+        cases.append((base, rng.randrange(0, 1 << 22)))
+    bound = Fraction(1, 1 << 53) * (1 + Fraction(1, 128))
+    exact = 0
+    for base, f in cases:
+        p = base + f
+        ip = emu.emu_recip_near(base, f)
+        err = abs(Fraction(ip) * p - 1)
+        assert err <= bound, (base, f, float(err))
+        exact += ip == 1.0 / p
+    assert exact > 0.9 * len(cases)      # informational: mostly the correctly rounded value
+
+
+def test_fp64_two_ladders_near(emu):
+    """The kernel's base-2 ladder pair above 2^40 with 1/p from f64_recip_near, on primes and
+    odd composites of segments at 2^40, 1e15 and just below 2^50, against Python's pow."""
+    rng = random.Random(11)
+    out = (ctypes.c_uint64 * 8)()
+    for base in (30 * ((1 << 40) // 30 + 1), 10 ** 15, 30 * (((1 << 50) - (1 << 22)) // 30)):
+        eb = (base + (1 << 22)).bit_length()
+        for _ in range(300):
+            f0, f1 = rng.randrange(1, 1 << 22, 2), rng.randrange(1, 1 << 22, 2)
+            emu.emu_fp64x2_near(base, f0, f1, eb, out)
+            check_pair(out, 2, base + f0, base + f1, eb)
