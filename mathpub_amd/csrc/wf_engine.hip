@@ -239,6 +239,33 @@ struct FillRun { u64 base; u32 seg0, nseg, lseg0, pad; };
 // nseg segments in all (<= RUN_MAX)
 struct FillDesc { u32 fr0, nfr, i0, i1, nseg, fp; };   // fp: every run lies below 2^53
 
+// A pass's fill plan as the host sends it, one array of u32 words: its fill groups (FillGroup,
+// 8 words each, in work-item order), the group of every 64th work item, then the slice tables
+// the groups share.  A group's primes [istart, bend) are cut at the ends words[slices],
+// words[slices + 1], ... (one table serves every group of one prime range and group size);
+// item0 is the index of its first work item.
+struct FillGroup { u32 fr0, nfr, nseg, fp, istart, bend, slices, item0; };
+static_assert(sizeof(FillGroup) == 8 * sizeof(u32), "a fill group is 8 words");
+
+// Work item b of a pass: its group from the group of item b & ~63 and at most 63 steps on
+// (almost always none), its slice from the group's table (the same item the host would have
+// listed).
+__device__ __forceinline__ FillDesc fill_item(const u32* __restrict__ fplan, u32 ngroups, u32 b) {
+  const FillGroup* __restrict__ g = reinterpret_cast<const FillGroup*>(fplan);
+  u32 lo = fplan[8 * ngroups + (b >> 6)];   // the last group whose first item is <= b
+  while (lo + 1 < ngroups && g[lo + 1].item0 <= b) lo++;
+  const FillGroup G = g[lo];
+  const u32 k = b - G.item0;
+  FillDesc it;
+  it.fr0 = G.fr0;
+  it.nfr = G.nfr;
+  it.nseg = G.nseg;
+  it.fp = G.fp;
+  it.i0 = k ? fplan[G.slices + k - 1] : G.istart;
+  it.i1 = min(fplan[G.slices + k], G.bend);
+  return it;
+}
+
 struct DevAcc {     // per range; every field is a 64-bit sum (checksums are per range and base)
   unsigned long long primes, tests, near, flt, nev, npsp, nbpsw;
 };
@@ -375,7 +402,7 @@ __device__ __forceinline__ void fill_walk(const FillDesc& it, const FillRun* __r
 // the count to the room left), and the workgroup copies them out with consecutive lanes on
 // consecutive addresses.  One workgroup per item: a grid of resident workgroups that loop
 // over the items measured 12% slower on the sieve stage.
-__global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __restrict__ items,
+__global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const u32* __restrict__ fplan, u32 ngroups,
                                                      const FillRun* __restrict__ fruns,
                                                      const u32* __restrict__ sp, const u64* __restrict__ spinv,
                                                      const double* __restrict__ sprcp, u32* __restrict__ bcnt, u32* __restrict__ bdata,
@@ -389,7 +416,7 @@ __global__ void __launch_bounds__(FILL_BLOCK) k_bucket_fill(const FillDesc* __re
   __shared__ WheelLds wl;
   if (SIEVE_PRIO) __builtin_amdgcn_s_setprio(SIEVE_PRIO);
   load_wheel(wl);
-  const FillDesc it = items[blockIdx.x];
+  const FillDesc it = fill_item(fplan, ngroups, blockIdx.x);
   const u32 smask = (1u << sbshift) - 1;
   const int cshift = 31 - __builtin_clz(FILL_STAGE / it.nseg);   // it.nseg <= RUN_MAX < FILL_STAGE
   const u32 C = 1u << cshift;
@@ -1500,13 +1527,13 @@ struct wf_engine {
   double* d_sprcp = nullptr;     // 1/q rounded to nearest (placement below 2^53)
   u32 tierA = 0, q1idx = 0;
   double surv_frac = 1.0;        // share of numbers with no prime factor below bucket_min
-  size_t fill_hint = 0;          // fill items of the last planned pass (a reserve size)
+  size_t fill_hint = 0;          // fill groups of the last planned pass (a reserve size)
   // pass descriptors (all passes of a batch) and two sets of pass buffers
   SegDesc* d_segs = nullptr;
   u64 seg_cap = 0;
   FillRun* d_runs = nullptr;     // fill runs of the pass(es)
   u64 run_cap = 0;
-  FillDesc* d_fills = nullptr;
+  u32* d_fills = nullptr;        // fill plans (FillGroup words and slice tables)
   u64 fill_cap = 0;
   BufSet bs[2];
   u32* d_flags = nullptr;
@@ -2027,7 +2054,8 @@ struct PassPlan {
   std::vector<SegDesc> segs;
   std::vector<RunDesc> runs;
   std::vector<FillRun> fruns;
-  std::vector<FillDesc> fills;
+  std::vector<u32> fills;          // the fill plan: groups, then slice tables (FillGroup)
+  u32 nfgroups = 0, nfitems = 0;   // fill groups and k_bucket_fill work items
   u32 maxnq = 0, bcap = 0, pcap = 0, tps = 0, nprove = 0;
 };
 
@@ -2100,7 +2128,12 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
   P.tps = (P.pcap + TEST_TILE - 1) / TEST_TILE;
   P.bcap = 0;
   P.fills.clear();
-  P.fills.reserve(e->fill_hint);
+  P.nfgroups = 0;
+  P.nfitems = 0;
+  std::vector<FillGroup> groups;
+  groups.reserve(e->fill_hint);
+  std::vector<u32> tables;                                // slice tables of the pass, each once
+  std::vector<std::pair<const std::vector<u32>*, u32>> toffs;   // table -> its offset in tables
   if (P.maxnq > e->q1idx) {
     // expected records per segment: sum over large primes and classes of seg_bytes / q; the
     // count is a sum of independent 0/1 (or floor/ceil) terms, so its deviation is ~sqrt(E)
@@ -2123,21 +2156,29 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
       u64 g_top = 0;   // end of the group's highest run
       auto flush = [&]() {
         if (!g_nseg) return;
-        const u32 bend = std::min(gend, g_nq);
+        const u32 bend = std::min(gend, g_nq);   // > i: every run of the group uses prime i
         const std::vector<u32>& ends = fill_slices(e, i, gend, g_nseg);
-        u32 a = i;
-        for (size_t k = 0; k < ends.size() && a < bend; k++) {
-          const u32 b = std::min(ends[k], bend);
-          FillDesc f;
-          f.fr0 = g_fr0;
-          f.nfr = (u32)P.fruns.size() - g_fr0;
-          f.i0 = a;
-          f.i1 = b;
-          f.nseg = g_nseg;
-          f.fp = g_top <= (1ull << 53) ? 1u : 0u;
-          P.fills.push_back(f);
-          a = b;
+        u32 toff = ~0u;
+        for (const auto& t : toffs)
+          if (t.first == &ends) toff = t.second;
+        if (toff == ~0u) {
+          toff = (u32)tables.size();
+          tables.insert(tables.end(), ends.begin(), ends.end());
+          toffs.emplace_back(&ends, toff);
         }
+        // the slices [i, ends[0]), [ends[0], ends[1]), ... that start below bend
+        const u32 nsl = 1 + (u32)(std::lower_bound(ends.begin(), ends.end() - 1, bend) - ends.begin());
+        FillGroup g;
+        g.fr0 = g_fr0;
+        g.nfr = (u32)P.fruns.size() - g_fr0;
+        g.nseg = g_nseg;
+        g.fp = g_top <= (1ull << 53) ? 1u : 0u;
+        g.istart = i;
+        g.bend = bend;
+        g.slices = toff;       // relative to the tables until they are placed after the groups
+        g.item0 = P.nfitems;
+        groups.push_back(g);
+        P.nfitems += nsl;
         g_fr0 = (u32)P.fruns.size();
         g_nseg = 0;
         g_nq = 0;
@@ -2165,7 +2206,17 @@ static void plan_pass(wf_engine* e, const std::vector<Piece>& pieces, PassPlan& 
       i = gend;
     }
   }
-  e->fill_hint = P.fills.size();
+  P.nfgroups = (u32)groups.size();
+  const u32 nidx = (P.nfitems + 63) / 64, tbase = 8 * P.nfgroups + nidx;
+  for (FillGroup& g : groups) g.slices += tbase;
+  P.fills.resize((size_t)tbase + tables.size());
+  if (!groups.empty()) memcpy(P.fills.data(), groups.data(), groups.size() * sizeof(FillGroup));
+  for (u32 k = 0, gi = 0; k < nidx; k++) {   // the group of item 64 k
+    while (gi + 1 < P.nfgroups && groups[gi + 1].item0 <= 64 * k) gi++;
+    P.fills[8 * P.nfgroups + k] = gi;
+  }
+  if (!tables.empty()) memcpy(P.fills.data() + tbase, tables.data(), tables.size() * sizeof(u32));
+  e->fill_hint = groups.size();
 }
 
 // Prove tiles per segment list.
@@ -2187,13 +2238,13 @@ static int size_set(BufSet& B, const PassPlan& P) {
 
 // Bucket fill + sieve of pass P (descriptors at dsegs / druns) into buffer set B on stream st.
 static int enqueue_sieve(wf_engine* e, const PassPlan& P, const SegDesc* dsegs, const FillRun* druns,
-                         const FillDesc* dfills, BufSet& B, hipStream_t st, DevAcc* acc, bool want_bitmap = false) {
+                         const u32* dfills, BufSet& B, hipStream_t st, DevAcc* acc, bool want_bitmap = false) {
   const u64 nseg = P.segs.size();
   const int cus = e->prop.multiProcessorCount;
   if (P.bcap) {
     hipLaunchKernelGGL(k_zero, dim3((u32)std::min<u64>((nseg + 255) / 256, 1024)), dim3(256), 0, st, B.bcnt, nseg);
-    if (!P.fills.empty()) {
-      hipLaunchKernelGGL(k_bucket_fill, dim3((u32)P.fills.size()), dim3(FILL_BLOCK), 0, st, dfills, druns, e->d_sp,
+    if (P.nfitems) {
+      hipLaunchKernelGGL(k_bucket_fill, dim3(P.nfitems), dim3(FILL_BLOCK), 0, st, dfills, P.nfgroups, druns, e->d_sp,
                          e->d_spinv, e->d_sprcp, B.bcnt, B.bdata, P.bcap, e->sbshift, e->d_flags);
       HIPCHK(hipGetLastError());
     }
@@ -2262,7 +2313,7 @@ static int upload_pass(wf_engine* e, const PassPlan& P) {
     HIPCHK(hipMemcpyAsync(e->d_runs, P.fruns.data(), P.fruns.size() * sizeof(FillRun), hipMemcpyHostToDevice, e->st));
   if (!P.fills.empty()) {
     if ((rc = grow(e->d_fills, e->fill_cap, P.fills.size())) != WF_OK) return rc;
-    HIPCHK(hipMemcpyAsync(e->d_fills, P.fills.data(), P.fills.size() * sizeof(FillDesc), hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipMemcpyAsync(e->d_fills, P.fills.data(), P.fills.size() * sizeof(u32), hipMemcpyHostToDevice, e->st));
   }
   return WF_OK;
 }
@@ -2350,7 +2401,7 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
     foff[k] = nfills;
     nsegs += plans[k].segs.size();
     nruns += plans[k].fruns.size();
-    nfills += plans[k].fills.size();
+    nfills += (plans[k].fills.size() + 7) & ~(size_t)7;   // each pass's plan 32-byte aligned
   }
   e->last.plan_ms += ms_since(t0);
   if (nsegs == 0) return WF_OK;
@@ -2360,7 +2411,7 @@ static int run_batched(wf_engine* e, const std::vector<std::pair<std::vector<Pie
   if ((rc = grow(e->d_fills, e->fill_cap, std::max<u64>(nfills, 1))) != WF_OK) return rc;
   for (size_t k = 0; k < plans.size(); k++) {
     if (!plans[k].fills.empty())
-      HIPCHK(hipMemcpyAsync(e->d_fills + foff[k], plans[k].fills.data(), plans[k].fills.size() * sizeof(FillDesc),
+      HIPCHK(hipMemcpyAsync(e->d_fills + foff[k], plans[k].fills.data(), plans[k].fills.size() * sizeof(u32),
                             hipMemcpyHostToDevice, e->st));
     if ((rc = size_set(e->bs[k & 1], plans[k])) != WF_OK) return rc;
     HIPCHK(hipMemcpyAsync(e->d_segs + soff[k], plans[k].segs.data(), plans[k].segs.size() * sizeof(SegDesc),
@@ -2956,7 +3007,7 @@ struct ServeStep {         // a step in flight
   u64 seg_cap = 0;
   FillRun* d_runs = nullptr;
   u64 run_cap = 0;
-  FillDesc* d_fills = nullptr;
+  u32* d_fills = nullptr;        // fill plans (FillGroup words and slice tables)
   u64 fill_cap = 0;
   DevEvent* d_ev = nullptr;
   unsigned long long* d_nev = nullptr;
@@ -3110,7 +3161,7 @@ static int serve_enqueue(wf_engine* e, Server* S, int ci, const std::vector<Piec
   for (const PassPlan& P : c.plans) {
     nsegs += P.segs.size();
     nruns += P.fruns.size();
-    nfills += P.fills.size();
+    nfills += (P.fills.size() + 7) & ~(size_t)7;   // each pass's plan 32-byte aligned
   }
   if ((rc = grow(c.d_segs, c.seg_cap, std::max<u64>(nsegs, 1))) != WF_OK) return rc;
   if ((rc = grow(c.d_runs, c.run_cap, std::max<u64>(nruns, 1))) != WF_OK) return rc;
@@ -3147,11 +3198,11 @@ static int serve_enqueue(wf_engine* e, Server* S, int ci, const std::vector<Piec
     if (!P.fruns.empty())
       HIPCHK(hipMemcpyAsync(c.d_runs + ro, P.fruns.data(), P.fruns.size() * sizeof(FillRun), hipMemcpyHostToDevice, e->st));
     if (!P.fills.empty())
-      HIPCHK(hipMemcpyAsync(c.d_fills + fo, P.fills.data(), P.fills.size() * sizeof(FillDesc), hipMemcpyHostToDevice,
+      HIPCHK(hipMemcpyAsync(c.d_fills + fo, P.fills.data(), P.fills.size() * sizeof(u32), hipMemcpyHostToDevice,
                             e->st));
     so += P.segs.size();
     ro += P.fruns.size();
-    fo += P.fills.size();
+    fo += (P.fills.size() + 7) & ~(size_t)7;
   }
   TestOut to;
   to.acc = S->d_acc;
