@@ -144,6 +144,7 @@ class Engine:
         self._h = h
         self._L = L
         self._bases = [2]
+        self._ev = None   # the event array the calls share (events are copied out before a call returns)
 
     def close(self):
         if self._h:
@@ -173,6 +174,12 @@ class Engine:
         _check(self._L.wf_set_bases(self._h, arr, len(bases)), "wf_set_bases")
         self._bases = list(bases)
 
+    def _evbuf(self, n):
+        n = max(n, 1)
+        if self._ev is None or len(self._ev) < n:
+            self._ev = (Event * n)()
+        return self._ev
+
     def _events(self, ev, n, with_range=False):
         if with_range:
             return [(e.type, e.p, e.q, (e.r_hi << 64) | e.r_lo, self._bases[e.j], e.range) for e in ev[:n]]
@@ -180,7 +187,7 @@ class Engine:
 
     def scan(self, lo, hi, near_T=0, ev_cap=1 << 16, psp=False, psp_cap=1 << 16):
         res = Result()
-        ev = (Event * max(ev_cap, 1))()
+        ev = self._evbuf(ev_cap)
         pbuf = (u64 * max(psp_cap, 1))()
         _check(self._L.wf_scan(self._h, lo, _hi(hi), near_T, ctypes.byref(res), ev, ev_cap, 1 if psp else 0,
                                pbuf, psp_cap), "wf_scan")
@@ -195,7 +202,7 @@ class Engine:
         n = len(ranges)
         rr = (Range * n)(*[Range(lo, _hi(hi)) for lo, hi in ranges])
         res = (Result * n)()
-        ev = (Event * max(ev_cap, 1))()
+        ev = self._evbuf(ev_cap)
         _check(self._L.wf_scan_batch(self._h, rr, n, near_T, res, ev, ev_cap), "wf_scan_batch")
         total_ev = sum(r.nev for r in res)
         events = self._events(ev, min(total_ev, ev_cap), with_range=True)
@@ -275,7 +282,7 @@ class Engine:
         """Result dict (with its events, as scan returns them) once the request is done, else
         None.  Raises WFError (WF_E_OVERFLOW) for a request whose events overflowed."""
         res, done = Result(), ctypes.c_int()
-        ev = (Event * max(ev_cap, 1))()
+        ev = self._evbuf(ev_cap)
         _check(self._L.wf_serve_poll(self._h, ticket, ctypes.byref(res), ev, ev_cap, ctypes.byref(done)),
                "wf_serve_poll")
         if not done.value:
