@@ -562,15 +562,19 @@ __device__ __forceinline__ void mont_powb(u64 p, u64 e, int ebits, u32 b, u32 (&
 // ---------------------------------------------------------------------------------
 // Limbs of 28 bits in 32-bit registers; every partial product (< 2^59) is one
 // v_mad_u64_u32 into a 64-bit column sum, and carries move once per column, so no
-// product needs its own carry chain.  N = p^2 < 2^128 leaves 12 spare bits below R:
-// any input x < 64 N squares to (x^2 + m N) / R < N (4096 N / R + 1) < 2N, so the base-2
-// ladder doubles without reducing (limbs may then hold 29 bits: products stay < 2^59 and
-// column sums < 2^62) and only the final REDC reduces fully.  This also covers p >= 2^63,
-// where N reaches 2^128 and a 32-bit-limb REDC needs its 129th bit.
+// product needs its own carry chain.  N = p^2 < 2^128 leaves 12 spare bits below R.
+// The squaring's REDC multipliers keep all 32 bits of m_j = T_j ninv (only their low 28
+// bits matter for clearing a column), so m = sum m_j 2^(28 j) < 2^144 (1 + 2^-27) and any
+// input x < 64 N squares to (x^2 + m N) / R < N (4096 N / R + 16.001) < 17.01 N.  The base-2
+// ladder therefore doubles without reducing (below 35 N; limbs 0..3 then hold 29 bits and
+// limb 4 22 bits, so the squaring's products stay < 2^59, the REDC products < 2^60 and every
+// column sum < 2^63), and only the final REDC, with 28-bit multipliers, reduces fully.  This
+// also covers p >= 2^63, where N reaches 2^128 and a 32-bit-limb REDC needs its 129th bit.
 
 #define M28 0x0FFFFFFFu
 
-// One REDC row: clear column T0 (mod 2^28) with m N, carry T0 into T1.
+// One REDC row: clear column T0 (mod 2^28) with m N, carry T0 into T1.  M28_ROW takes m
+// below 2^28 (the final, full reduction); M28_ROW_LAZY keeps m's top four bits.
 #define M28_ROW(T0, T1, T2, T3, T4)                  \
   m = ((u32)T0 * ninv) & M28;                      \
   T0 += (u64)m * n[0];                             \
@@ -578,8 +582,15 @@ __device__ __forceinline__ void mont_powb(u64 p, u64 e, int ebits, u32 b, u32 (&
   T2 += (u64)m * n[2];                             \
   T3 += (u64)m * n[3];                             \
   T4 += (u64)m * n[4];
+#define M28_ROW_LAZY(T0, T1, T2, T3, T4)             \
+  m = (u32)T0 * ninv;                              \
+  T0 += (u64)m * n[0];                             \
+  T1 += (u64)m * n[1] + (T0 >> 28);                \
+  T2 += (u64)m * n[2];                             \
+  T3 += (u64)m * n[3];                             \
+  T4 += (u64)m * n[4];
 
-// x = x^2 / R mod N (lazy: input < 64 N, result < 2N with normalised limbs)
+// x = x^2 / R mod N (lazy: input < 64 N, result < 17.01 N with limbs 0..3 below 2^28)
 __device__ __forceinline__ void m28_sqr(u32 (&x)[5], const u32 (&n)[5], u32 ninv) {
   const u32 d1 = x[1] << 1, d2 = x[2] << 1, d3 = x[3] << 1, d4 = x[4] << 1;
   u64 t0 = (u64)x[0] * x[0];
@@ -592,11 +603,11 @@ __device__ __forceinline__ void m28_sqr(u32 (&x)[5], const u32 (&n)[5], u32 ninv
   u64 t7 = (u64)x[3] * d4;
   u64 t8 = (u64)x[4] * x[4];
   u32 m;
-  M28_ROW(t0, t1, t2, t3, t4)
-  M28_ROW(t1, t2, t3, t4, t5)
-  M28_ROW(t2, t3, t4, t5, t6)
-  M28_ROW(t3, t4, t5, t6, t7)
-  M28_ROW(t4, t5, t6, t7, t8)
+  M28_ROW_LAZY(t0, t1, t2, t3, t4)
+  M28_ROW_LAZY(t1, t2, t3, t4, t5)
+  M28_ROW_LAZY(t2, t3, t4, t5, t6)
+  M28_ROW_LAZY(t3, t4, t5, t6, t7)
+  M28_ROW_LAZY(t4, t5, t6, t7, t8)
   x[0] = (u32)t5 & M28;
   t6 += t5 >> 28;
   x[1] = (u32)t6 & M28;
@@ -631,6 +642,7 @@ __device__ __forceinline__ void m28_out(const u32 (&x)[5], const u32 (&n)[5], u3
   r[3] = (u32)(z >> 96);
 }
 #undef M28_ROW
+#undef M28_ROW_LAZY
 
 // N = p^2 as five 28-bit limbs, -N^-1 mod 2^28, R mod N (R = 2^140) and 1/N in FP64.
 struct M28Mod {
@@ -669,28 +681,6 @@ __device__ __forceinline__ void m28_setup(u64 p, M28Mod& m) {
   m.rmodn = vlow;
 }
 
-// r = 2^e mod N = p^2 as four 32-bit words, for odd p >= 2^47.  ebits = the wave's longest
-// exponent length (shorter exponents skip their leading positions).
-__device__ __forceinline__ void m28_pow2(u64 p, u64 e, int ebits, u32 (&r)[4]) {
-  typedef unsigned __int128 u128;
-  M28Mod m;
-  m28_setup(p, m);
-  // Montgomery form of 2: 2 (R mod N) < 2N
-  const u128 two = m.rmodn << 1;                      // < 2^129: bit 128 is lost only if N >= 2^127
-  u32 x[5];
-  for (int i = 0; i < 4; i++) x[i] = (u32)(two >> (28 * i)) & M28;
-  x[4] = (u32)(two >> 112) | ((m.rmodn >> 127) ? (1u << 16) : 0u);
-  // ladder over the bits below this lane's top bit
-  const int tb = 63 - __builtin_clzll(e);
-  for (int i = ebits - 2; i >= 0; i--) {
-    if (i >= tb) continue;
-    m28_sqr(x, m.n, m.ninv);
-    const u32 bit = (u32)(e >> i) & 1u;
-    x[0] <<= bit; x[1] <<= bit; x[2] <<= bit; x[3] <<= bit; x[4] <<= bit;
-  }
-  m28_out(x, m.n, m.ninv, r);
-}
-
 // x = x c mod N for an integer c < 2^32 (x < 64 N on entry, < 3 N on exit).  The
 // Montgomery form of a product with a plain integer is that form times the integer, so no
 // REDC is needed: T = x c < 2^38 N, its quotient by N is estimated in FP64 from the top
@@ -720,6 +710,27 @@ __device__ __forceinline__ void m28_mul_small(u32 (&x)[5], u32 c, const M28Mod& 
   x[2] = (u32)c2 & M28;
   x[3] = (u32)c3 & M28;
   x[4] = (u32)c4 & M28;
+}
+
+// r = 2^e mod N = p^2 as four 32-bit words, for odd p >= 2^47.  ebits = the wave's longest
+// exponent length (shorter exponents skip their leading positions).  The exponent's five
+// leading bits t (16 <= t < 32) enter at once: the Montgomery form of 2^t is (R mod N) 2^t,
+// one m28_mul_small in place of four squarings.  Each later bit is a squaring and, for a
+// one, a doubling by limb shifts (below 35 N, inside the squaring's 64 N).
+__device__ __forceinline__ void m28_pow2(u64 p, u64 e, int ebits, u32 (&r)[4]) {
+  M28Mod m;
+  m28_setup(p, m);
+  u32 x[5];                                          // Montgomery form of 1: R mod N < N
+  for (int i = 0; i < 5; i++) x[i] = (u32)(m.rmodn >> (28 * i)) & M28;
+  const int tb = 63 - __builtin_clzll(e);            // >= 46 for p >= 2^47
+  m28_mul_small(x, 1u << (u32)(e >> (tb - 4)), m);   // 2^t R mod N, < 3N
+  for (int i = ebits - 6; i >= 0; i--) {
+    if (i > tb - 5) continue;
+    m28_sqr(x, m.n, m.ninv);
+    const u32 bit = (u32)(e >> i) & 1u;
+    x[0] <<= bit; x[1] <<= bit; x[2] <<= bit; x[3] <<= bit; x[4] <<= bit;
+  }
+  m28_out(x, m.n, m.ninv, r);
 }
 
 // r = b^e mod N = p^2 (four 32-bit words) for a base with window W: pw[d] = b^d < 2^32.

@@ -140,6 +140,16 @@ void emu_m28(uint64_t p, int ebits, uint64_t* hi, uint64_t* lo) {
   *hi = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
 }
 
+// One lazy squaring x^2 / R mod N on limbs as the ladder holds them (x[i] may exceed 28 bits)
+void emu_m28_sqr(uint64_t p, const uint32_t* xin, uint32_t* xout) {
+  M28Mod m;
+  m28_setup(p, m);
+  uint32_t x[5];
+  for (int i = 0; i < 5; i++) x[i] = xin[i];
+  m28_sqr(x, m.n, m.ninv);
+  for (int i = 0; i < 5; i++) xout[i] = x[i];
+}
+
 // The order sieve of the SPRP-2 pass over the odd numbers A, A + 2, ..., A + 2 (nodd - 1):
 // out[i] = 1 where some prime 101 <= p < pmax rules A + 2 i out (psp_order_entry and
 // psp_order_walk, as k_psp runs them).

@@ -39,6 +39,7 @@ def emu():
     L.emu_recip_near.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
     L.emu_recip_near.restype = ctypes.c_double
     L.emu_fp64x2_near.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, P]
+    L.emu_m28_sqr.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     return L
 
 
@@ -187,6 +188,41 @@ def test_radix28_montgomery(emu):
         eb = (p - 1).bit_length()
         assert call(emu.emu_m28, p, eb) == pow(2, p - 1, p * p), p
         assert call(emu.emu_m28, p, min(64, eb + 2)) == pow(2, p - 1, p * p), p
+
+
+def test_radix28_lazy_squaring_bounds(emu):
+    """m28_sqr on the largest limbs the ladder can hand it: a value below 17.01 N with 28-bit
+    limbs 0..3, doubled (limbs of 29 bits, below 34.02 N), and values up to 64 N.  The result
+    must be x^2 / R mod N, below 17.01 N, with limbs 0..3 below 2^28: the bound that keeps the
+    next squaring's column sums inside 64 bits and lets the final REDC reduce in one step."""
+    R = 1 << 140
+    M = (1 << 28) - 1
+    rng = random.Random(28)
+    ps = [(1 << 64) - 59, (1 << 64) - 1, (1 << 63) + 29, (1 << 50) + 27, (1 << 47) + 1] + samples(47, 64, 300, 5)
+    for p in ps:
+        N = p * p
+        rinv = pow(R, -1, N)
+        tops = [17 * N + N // 100, 34 * N + N // 50, 64 * N - 1]
+        cases = []
+        for top in tops:
+            cases.append(top)
+            cases += [rng.randrange(top) for _ in range(6)]
+        for X in cases:
+            if X >= 34 * N and X < 64 * N - 1 and rng.random() < 0.5:
+                continue
+            limbs = [(X >> (28 * i)) & M for i in range(4)] + [X >> 112]
+            if X <= 34 * N + N // 50 and X % 2 == 0 and rng.random() < 0.5:
+                # the doubled form the ladder really holds: each limb twice a 28-bit limb
+                h = X // 2
+                limbs = [2 * ((h >> (28 * i)) & M) for i in range(4)] + [2 * (h >> 112)]
+            assert sum(v << (28 * i) for i, v in enumerate(limbs)) == X
+            xin = (ctypes.c_uint32 * 5)(*limbs)
+            xout = (ctypes.c_uint32 * 5)()
+            emu.emu_m28_sqr(p, xin, xout)
+            Y = sum(xout[i] << (28 * i) for i in range(5))
+            assert all(xout[i] <= M for i in range(4)), (p, X)
+            assert Y % N == X * X * rinv % N, (p, X)
+            assert Y < 17.01 * N, (p, X, Y / N)
 
 
 def test_psp_filter_keeps_every_strong_pseudoprime(emu):
