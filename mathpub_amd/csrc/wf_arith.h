@@ -907,6 +907,20 @@ __device__ __forceinline__ bool psp_possible(u64 n) {
   return ok && (vm & (vm - 1u)) == 0u;
 }
 
+// The Montgomery form (R = 2^64) of 2^t for 0 <= t < 32: (R mod n) 2^t mod n.  The quotient
+// (below 2^31) comes from an FP64 estimate within one of the truth (relative error below
+// 2^-51); one less is subtracted in 128-bit integers, leaving [0, 3n), then at most two
+// subtractions of n.
+__device__ __forceinline__ u64 mont64_pow2_form(u64 one, u64 n, int t) {
+  typedef unsigned __int128 u128;
+  double qd = __builtin_floor(__builtin_ldexp((double)one, t) / (double)n);
+  qd = qd >= 1.0 ? qd - 1.0 : 0.0;
+  u128 r = ((u128)one << t) - (u128)(u64)qd * n;
+  r = r >= n ? r - n : r;
+  r = r >= n ? r - n : r;
+  return (u64)r;
+}
+
 // true iff n is a base-2 strong probable prime (n odd, n >= 3)
 __device__ __forceinline__ bool sprp2_64(u64 n) {
   const u64 ninv = 0 - inv64(n);
@@ -917,9 +931,12 @@ __device__ __forceinline__ bool sprp2_64(u64 n) {
   u64 one = 1ull << (nb - 1);
   for (int k = nb - 1; k < 64; k++) one = mont64_dbl(one, n);
   const u64 mone = n - one;                       // Montgomery form of n - 1
-  // x = 2^d in Montgomery form, left to right from the top bit of d
-  u64 x = mont64_dbl(one, n);                     // 2
-  for (int i = 62 - __builtin_clzll(d); i >= 0; i--) {
+  // x = 2^d in Montgomery form, left to right: d's five leading bits at once, then one
+  // squaring (and for a one, a doubling) per further bit
+  const int db = 64 - __builtin_clzll(d);
+  const int sh = db > 5 ? db - 5 : 0;
+  u64 x = mont64_pow2_form(one, n, (int)(d >> sh));
+  for (int i = sh - 1; i >= 0; i--) {
     x = mont64_sqr(x, n, ninv);
     if ((d >> i) & 1) x = mont64_dbl(x, n);
   }

@@ -1291,8 +1291,9 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
                                              u64* __restrict__ out, u64 cap, unsigned long long* __restrict__ count,
                                              DevAcc* __restrict__ acc, unsigned long long* __restrict__ ntests) {
   __shared__ u64 queue[PSP_TILE];
+  __shared__ unsigned short surv[PSP_TILE];   // odd composites the order sieve leaves (tile index)
   __shared__ u32 ruled[PSP_TILE / 32];   // bit i: odd number i of the tile ruled out by the order sieve
-  __shared__ u32 qn;
+  __shared__ u32 qn, sn;
   __shared__ signed char w30[32];
   if (threadIdx.x < 30) w30[threadIdx.x] = c_W30[threadIdx.x];   // ordered by the first barrier below
   const int lane = threadIdx.x & 63;
@@ -1303,7 +1304,7 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
     const SegDesc d = segs[sg];
     const u32 nodd = 15u * d.nbytes;                 // odd offsets 1, 3, ..., 30 nbytes - 1
     if (k * PSP_TILE >= nodd) continue;              // uniform across the workgroup
-    if (threadIdx.x == 0) qn = 0;
+    if (threadIdx.x == 0) qn = sn = 0;
     for (u32 w = threadIdx.x; w < PSP_TILE / 32; w += blockDim.x) ruled[w] = 0;
     __syncthreads();
     // order sieve over the tile's odd numbers (psp_order_walk per prime)
@@ -1313,27 +1314,42 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
         psp_order_walk(A, po[f], PSP_TILE, [&](u32 i) { atomicOr(&ruled[i >> 5], 1u << (i & 31u)); });
     }
     __syncthreads();
-    for (u32 j0 = 0; j0 < PSP_TILE; j0 += blockDim.x) {
-      const u32 li = j0 + threadIdx.x;
-      const u32 i = k * PSP_TILE + li;
-      const u32 off = 2u * i + 1u;
-      const u64 n = d.base + off;
-      bool want = false;
-      if (i < nodd && off >= d.lo_off && off < d.hi_off && !((ruled[li >> 5] >> (li & 31u)) & 1u)) {
-        const int cls = w30[off % 30u];
-        const bool comp = cls < 0 ? n > 5
-                                  : ((bitmap[(u64)sg * bmw + off / 120u] >> (((off / 30u) & 3u) * 8u + (u32)cls)) & 1u);
-        want = comp && psp_possible(n);
-      }
-      // one LDS atomic per wavefront reserves the queue slots of its candidates
+    // one LDS atomic per wavefront reserves the list slots of its lanes' entries
+    auto push = [&](bool want, u32& cnt, auto store) {
       const u64 mask = __ballot(want);
       if (mask) {
         const int leader = __ffsll((unsigned long long)mask) - 1;
         u32 base = 0;
-        if (lane == leader) base = atomicAdd(&qn, (u32)__popcll(mask));
+        if (lane == leader) base = atomicAdd(&cnt, (u32)__popcll(mask));
         base = __shfl(base, leader, 64);
-        if (want) queue[base + __popcll(mask & ((1ull << lane) - 1))] = n;
+        if (want) store(base + __popcll(mask & ((1ull << lane) - 1)));
       }
+    };
+    // the odd composites left by the order sieve, compacted, so that the filter below runs
+    // with every lane busy (about half the tile is ruled out)
+    for (u32 j0 = 0; j0 < PSP_TILE; j0 += blockDim.x) {
+      const u32 li = j0 + threadIdx.x;
+      const u32 i = k * PSP_TILE + li;
+      const u32 off = 2u * i + 1u;
+      bool keep = false;
+      if (i < nodd && off >= d.lo_off && off < d.hi_off && !((ruled[li >> 5] >> (li & 31u)) & 1u)) {
+        const int cls = w30[off % 30u];
+        keep = cls < 0 ? d.base + off > 5
+                       : ((bitmap[(u64)sg * bmw + off / 120u] >> (((off / 30u) & 3u) * 8u + (u32)cls)) & 1u);
+      }
+      push(keep, sn, [&](u32 slot) { surv[slot] = (unsigned short)li; });
+    }
+    __syncthreads();
+    const u32 ns = sn;
+    for (u32 j0 = 0; j0 < ns; j0 += blockDim.x) {    // uniform bound: every lane reaches the ballot
+      const u32 j = j0 + threadIdx.x;
+      u64 n = 0;
+      bool want = false;
+      if (j < ns) {
+        n = d.base + 2ull * (k * PSP_TILE + surv[j]) + 1;
+        want = psp_possible(n);
+      }
+      push(want, qn, [&](u32 slot) { queue[slot] = n; });
     }
     __syncthreads();
     const u32 m = qn;

@@ -173,10 +173,16 @@ def test_sprp2_64(emu):
                                      3825123056546413051, 318665857834031151167461 % (1 << 64) | 1]
     rng = random.Random(3)
     ns += [rng.getrandbits(64) | 1 for _ in range(3000)] + [rng.getrandbits(40) | 1 for _ in range(1000)]
+    # every size, so that d's five leading bits (the FP64-estimated start) take every form
+    ns += [rng.getrandbits(rng.randint(2, 64)) | 1 for _ in range(3000)]
+    # long runs of squarings after the ladder (n - 1 = d 2^s, s up to 62), primes among them
+    ns += [(rng.getrandbits(max(1, 63 - s)) << s) + 1 for s in range(1, 63) for _ in range(20)]
+    ns += [3 * (1 << 30) + 1, 15 * (1 << 27) + 1, 18446744066047760377, 18446743208455367653]
     for n in ns:
-        if n < 3:
+        if n < 3 or n % 2 == 0 or n >= 1 << 64:
             continue
         assert bool(emu.emu_sprp2(n)) == strong2(n), n
+    assert sum(strong2(n) for n in ns if 3 <= n < 1 << 64 and n % 2) > 2000   # many true cases
 
 
 def test_radix28_montgomery(emu):
