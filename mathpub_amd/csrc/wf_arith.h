@@ -857,17 +857,16 @@ __device__ __forceinline__ bool divides32(u32 x) {
 // mod the odd prime powers in ord_r(2); those modulo 4, 8 and 16 are n's low bits.  The
 // 2-adic valuations of the orders of the factors found are collected as bits of vm, which
 // must hold at most one bit.  Every lane runs the same instructions, so a wavefront never
-// splits here.
-__device__ __forceinline__ bool psp_possible(u64 n) {
-  constexpr u32 M1 = 9u * 5 * 7 * 11 * 13 * 17 * 19 * 23;   // 334639305
-  constexpr u32 M2 = 29u * 31 * 37 * 41 * 43;               // 58642669
-  constexpr u32 M3 = 47u * 53 * 59 * 61 * 67;               // 600662303
-  constexpr u32 M4 = 71u * 73 * 79 * 83 * 89;               // 3024658859
-  const u32 g1 = (u32)(n % M1), g2 = (u32)(n % M2), g3 = (u32)(n % M3), g4 = (u32)(n % M4);
-  const u32 g5 = (u32)(n % 97u);
+// splits here.  psp_possible_res takes n's residues (gi = n mod PSP_Mi, g5 = n mod 97) and
+// its low four bits n4, which a caller walking consecutive numbers can advance cheaply.
+constexpr u32 PSP_M1 = 9u * 5 * 7 * 11 * 13 * 17 * 19 * 23;   // 334639305
+constexpr u32 PSP_M2 = 29u * 31 * 37 * 41 * 43;               // 58642669
+constexpr u32 PSP_M3 = 47u * 53 * 59 * 61 * 67;               // 600662303
+constexpr u32 PSP_M4 = 71u * 73 * 79 * 83 * 89;               // 3024658859
+__device__ __forceinline__ bool psp_possible_res(u32 n4, u32 g1, u32 g2, u32 g3, u32 g4, u32 g5) {
+  constexpr u32 M1 = PSP_M1, M2 = PSP_M2;
   // n = 1 modulo 4, 8, 16 and modulo odd divisors of M1 and M2 (g + M - 1 < 2^32 avoids the
   // wrap of g - 1 at g = 0)
-  const u32 n4 = (u32)n & 15u;
   const bool e4 = (n4 & 3u) == 1u, e8 = (n4 & 7u) == 1u, e16 = n4 == 1u;
   const u32 h1 = g1 + (M1 - 1u), h2 = g2 + (M2 - 1u);
   const bool e3 = divides32<3>(h1), e9 = divides32<9>(h1), e5 = divides32<5>(h1), e7 = divides32<7>(h1),
@@ -907,15 +906,20 @@ __device__ __forceinline__ bool psp_possible(u64 n) {
   return ok && (vm & (vm - 1u)) == 0u;
 }
 
-// The Montgomery form (R = 2^64) of 2^t for 0 <= t < 32: (R mod n) 2^t mod n.  The quotient
-// (below 2^31) comes from an FP64 estimate within one of the truth (relative error below
-// 2^-51); one less is subtracted in 128-bit integers, leaving [0, 3n), then at most two
-// subtractions of n.
-__device__ __forceinline__ u64 mont64_pow2_form(u64 one, u64 n, int t) {
+__device__ __forceinline__ bool psp_possible(u64 n) {
+  return psp_possible_res((u32)n & 15u, (u32)(n % PSP_M1), (u32)(n % PSP_M2), (u32)(n % PSP_M3), (u32)(n % PSP_M4),
+                          (u32)(n % 97u));
+}
+
+// x 2^k mod n for x < n and 0 <= k < 32 (a Montgomery form times 2^k is the form of the
+// product).  The quotient (below 2^31) comes from an FP64 estimate with invn = 1/n rounded
+// (relative error below 2^-51, so within one of the truth); one less is subtracted in
+// 128-bit integers, leaving [0, 3n), then at most two subtractions of n.
+__device__ __forceinline__ u64 mul_pow2_mod(u64 x, int k, u64 n, double invn) {
   typedef unsigned __int128 u128;
-  double qd = __builtin_floor(__builtin_ldexp((double)one, t) / (double)n);
+  double qd = __builtin_floor(__builtin_ldexp((double)x, k) * invn);
   qd = qd >= 1.0 ? qd - 1.0 : 0.0;
-  u128 r = ((u128)one << t) - (u128)(u64)qd * n;
+  u128 r = ((u128)x << k) - (u128)(u64)qd * n;
   r = r >= n ? r - n : r;
   r = r >= n ? r - n : r;
   return (u64)r;
@@ -924,6 +928,7 @@ __device__ __forceinline__ u64 mont64_pow2_form(u64 one, u64 n, int t) {
 // true iff n is a base-2 strong probable prime (n odd, n >= 3)
 __device__ __forceinline__ bool sprp2_64(u64 n) {
   const u64 ninv = 0 - inv64(n);
+  const double invn = 1.0 / (double)n;
   const int s = __builtin_ctzll(n - 1);
   const u64 d = (n - 1) >> s;
   // one = R mod n: 2^(bits(n) - 1) doubled up to 2^64
@@ -931,14 +936,16 @@ __device__ __forceinline__ bool sprp2_64(u64 n) {
   u64 one = 1ull << (nb - 1);
   for (int k = nb - 1; k < 64; k++) one = mont64_dbl(one, n);
   const u64 mone = n - one;                       // Montgomery form of n - 1
-  // x = 2^d in Montgomery form, left to right: d's five leading bits at once, then one
-  // squaring (and for a one, a doubling) per further bit
+  // x = 2^d in Montgomery form, left to right in 5-bit digits of d (aligned to its bit 0):
+  // the top digit t gives (R mod n) 2^t, each further digit five squarings and one
+  // multiplication by 2^digit
   const int db = 64 - __builtin_clzll(d);
-  const int sh = db > 5 ? db - 5 : 0;
-  u64 x = mont64_pow2_form(one, n, (int)(d >> sh));
-  for (int i = sh - 1; i >= 0; i--) {
-    x = mont64_sqr(x, n, ninv);
-    if ((d >> i) & 1) x = mont64_dbl(x, n);
+  int sh = (db - 1) / 5 * 5;
+  u64 x = mul_pow2_mod(one, (int)(d >> sh), n, invn);
+  for (sh -= 5; sh >= 0; sh -= 5) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) x = mont64_sqr(x, n, ninv);
+    x = mul_pow2_mod(x, (int)((d >> sh) & 31u), n, invn);
   }
   if (x == one || x == mone) return true;
   for (int r = 1; r < s; r++) {

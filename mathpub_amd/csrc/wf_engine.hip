@@ -1279,6 +1279,8 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
 #ifndef PSP_WAVES_EU
 #define PSP_WAVES_EU 6
 #endif
+static_assert((u64)PSP_M4 + 2u * PSP_TILE < (1ull << 32) && 2u * PSP_TILE < PSP_M2,
+              "a tile's offsets 2 li stay below PSP_M1..PSP_M4 (the smallest is PSP_M2) and keep r + 2 li in 32 bits");
 #ifndef PSP_ORDER_MAX
 #define PSP_ORDER_MAX 16384  // the order sieve takes the primes 101 <= p < PSP_ORDER_MAX (< 2^15)
 #endif
@@ -1341,13 +1343,21 @@ __global__ void __launch_bounds__(256) k_psp(const SegDesc* __restrict__ segs, u
     }
     __syncthreads();
     const u32 ns = sn;
+    // the tile's numbers are A + 2 li: their residues for psp_possible are A's (once per
+    // tile) plus 2 li < 2^12, reduced by one subtraction
+    const u64 A = d.base + 2ull * k * PSP_TILE + 1;
+    const u32 r1 = (u32)(A % PSP_M1), r2 = (u32)(A % PSP_M2), r3 = (u32)(A % PSP_M3), r4 = (u32)(A % PSP_M4);
+    const u32 r5 = (u32)(A % 97u);
     for (u32 j0 = 0; j0 < ns; j0 += blockDim.x) {    // uniform bound: every lane reaches the ballot
       const u32 j = j0 + threadIdx.x;
       u64 n = 0;
       bool want = false;
       if (j < ns) {
-        n = d.base + 2ull * (k * PSP_TILE + surv[j]) + 1;
-        want = psp_possible(n);
+        const u32 o = 2u * surv[j];
+        n = A + o;
+        const u32 g1 = r1 + o, g2 = r2 + o, g3 = r3 + o, g4 = r4 + o;
+        want = psp_possible_res((u32)n & 15u, g1 >= PSP_M1 ? g1 - PSP_M1 : g1, g2 >= PSP_M2 ? g2 - PSP_M2 : g2,
+                                g3 >= PSP_M3 ? g3 - PSP_M3 : g3, g4 >= PSP_M4 ? g4 - PSP_M4 : g4, (r5 + o) % 97u);
       }
       push(want, qn, [&](u32 slot) { queue[slot] = n; });
     }

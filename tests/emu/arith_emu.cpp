@@ -126,6 +126,24 @@ int emu_sprp2(uint64_t n) { return sprp2_64(n) ? 1 : 0; }
 
 int emu_psp_possible(uint64_t n) { return psp_possible(n) ? 1 : 0; }
 
+// psp_possible from a tile's residues advanced by offsets, as k_psp forms them (n = A + o,
+// o even and below 2^12): the number of n in A, A + 2, ..., A + 2 (count - 1) where the two
+// forms disagree
+uint32_t emu_psp_tile_mismatch(uint64_t A, uint32_t count) {
+  const uint32_t r1 = (uint32_t)(A % PSP_M1), r2 = (uint32_t)(A % PSP_M2), r3 = (uint32_t)(A % PSP_M3),
+                 r4 = (uint32_t)(A % PSP_M4), r5 = (uint32_t)(A % 97u);
+  uint32_t bad = 0;
+  for (uint32_t i = 0; i < count; i++) {
+    const uint32_t o = 2u * i;
+    const uint64_t n = A + o;
+    const uint32_t g1 = r1 + o, g2 = r2 + o, g3 = r3 + o, g4 = r4 + o;
+    const bool t = psp_possible_res((uint32_t)n & 15u, g1 >= PSP_M1 ? g1 - PSP_M1 : g1, g2 >= PSP_M2 ? g2 - PSP_M2 : g2,
+                                    g3 >= PSP_M3 ? g3 - PSP_M3 : g3, g4 >= PSP_M4 ? g4 - PSP_M4 : g4, (r5 + o) % 97u);
+    bad += t != psp_possible(n);
+  }
+  return bad;
+}
+
 int emu_slprp(uint64_t n) { return slprp_64(n) ? 1 : 0; }
 
 

@@ -231,6 +231,24 @@ def test_radix28_lazy_squaring_bounds(emu):
             assert Y < 17.01 * N, (p, X, Y / N)
 
 
+def test_psp_filter_from_tile_residues(emu):
+    """k_psp forms psp_possible's residues from the tile's first number plus the offset; the
+    verdicts equal psp_possible(n) on whole tiles, including tiles whose residues wrap."""
+    emu.emu_psp_tile_mismatch.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
+    emu.emu_psp_tile_mismatch.restype = ctypes.c_uint32
+    rng = random.Random(97)
+    starts = [3, (1 << 64) - 4097, 18446744066047760377 - 2 * 100, 2047 - 20] + \
+        [rng.getrandbits(64) | 1 for _ in range(40)] + [rng.getrandbits(40) | 1 for _ in range(20)]
+    # tiles whose residues wrap modulo each product inside the tile
+    for M in (334639305, 58642669, 600662303, 3024658859, 97):
+        base = rng.getrandbits(60) * M
+        starts.append((base - 2 * 777) | 1)
+    for A in starts:
+        if A + 2 * 2048 >= 1 << 64:
+            A = (1 << 64) - 2 * 2048 - 1
+        assert emu.emu_psp_tile_mismatch(A, 2048) == 0, A
+
+
 def test_psp_filter_keeps_every_strong_pseudoprime(emu):
     """psp_possible is a necessary condition: it holds for every base-2 strong pseudoprime
     below 2e5 (brute force), for the one found below 2^64 in the top-of-word slice, and
