@@ -1268,11 +1268,12 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
   rlo_out[i] = rlo;
 }
 
-// Base-2 strong pseudoprimes: every odd composite n of the segments (wheel composites
-// from the sieve bitmap, and the odd multiples of 3 and 5) that passes the small-factor
-// condition psp_possible is queued in LDS; the workgroup then runs the exact SPRP-2 test
-// on the queue with every lane busy, and the n that pass are appended to out[] and counted
-// per range.
+// Base-2 strong pseudoprimes, per tile of PSP_TILE odd numbers: the order sieve rules out
+// odd multiples of 101 <= p < PSP_ORDER_MAX off their order class; the odd composites left
+// (wheel composites from the sieve bitmap, and the odd multiples of 3 and 5) are listed in
+// LDS; those of the list that pass the small-factor condition psp_possible are queued; the
+// workgroup then runs the exact SPRP-2 test on the queue with every lane busy, and the n
+// that pass are appended to out[] and counted per range.
 #ifndef PSP_TILE
 #define PSP_TILE 2048
 #endif
