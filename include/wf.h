@@ -98,7 +98,7 @@ typedef struct {
                                fewer leave room for the next pass's sieve beside the tests */
     uint32_t prove;         /* partial sieve + proof for short pieces: WF_PROVE_AUTO (by a cost
                                model), WF_PROVE_OFF, WF_PROVE_ON (every piece above bucket_min^2) */
-    uint32_t derive;        /* composite bases from their prime factors' quotients (p < 2^50):
+    uint32_t derive;        /* composite bases from their prime factors' quotients (p < 9 * 2^47):
                                WF_DERIVE_AUTO (whenever that needs fewer ladders than bases),
                                WF_DERIVE_OFF (one ladder per base) */
     uint32_t fault_step;    /* testing only, 0 = off: serving step number fault_step (counted

@@ -4,7 +4,7 @@
 // Two exact implementations of r = 2^(p-1) mod p^2 (A001220/wieferich.c:893-949 computes
 // the same residue with 2 x 64-bit Montgomery limbs and a 5-bit window):
 //
-//  * fp64_pow2 / fp64_pow2x2: p < 2^50.  The residue is held in p-adic digits x = a + b p
+//  * fp64_pow2 / fp64_pow2x2: p < F64_PMAX = 9 * 2^47.  The residue is held in p-adic digits x = a + b p
 //    with a, b signed integers stored exactly in doubles.  A squaring needs a^2 mod p,
 //    floor(a^2/p) and 2ab mod p; each product is split exactly by FMA (h = x*y,
 //    l = fma(x, y, -h)) and reduced with a rounded quotient k = rint(h / p) and an exact FMA
@@ -51,10 +51,11 @@ __device__ __forceinline__ u64 inv64(u64 p) {
 }
 
 // ---------------------------------------------------------------------------------
-// FP64 p-adic path, p < 2^50
+// FP64 p-adic path, p < F64_PMAX = 9 * 2^47 (about 1.267e15)
 // ---------------------------------------------------------------------------------
 //
-// Invariants (u = 2^-53, p < 2^50, all values integers held exactly):
+// Invariants for p < 2^50 (u = 2^-53, all values integers held exactly), for any number of
+// squarings:
 //   |a| <= 0.59 p (0.5 p right after a window step), |b| <= 1.9 p.
 // Squaring, x^2 = a^2 + 2ab p (mod p^2), quotients rounded once inside an FMA:
 //   a^2 = k1 p + a'      k1 = round(fl(a*a) / p): |k1 - a^2/p| <= 0.5 + 0.25 alpha^2,
@@ -66,6 +67,14 @@ __device__ __forceinline__ u64 inv64(u64 p) {
 // 2^52, so each FMA result is exact.  ip may be 1/p correctly rounded or within
 // 2^-53 (1 + 2^-7) of it (f64_recip_near): the 0.25 alpha^2 terms grow by 2^-8 of themselves,
 // and the fixed points above keep their digits.
+// From 2^50 to F64_PMAX the quotient errors grow with c = p / 2^52 (0.25 alpha^2 becomes
+// about c alpha^2) and the iteration has no fixed point below 2 for beta, but a ladder never
+// runs more than five squarings between window steps, and a window step (f64_mul_pow2*,
+// f64_mul_small*) leaves |a| <= p/2 (1 + 2^-20), |b| <= 1.1 p.  From there, at c = 0.28125:
+// alpha 0.571, 0.592, 0.599, 0.601, 0.602 and beta 1.56, 1.83, 1.96, 2.02, 2.05 after one to
+// five squarings, the largest rounded quotient is 0.69 * 2^51 and |b| stays below 0.29 * 2^53
+// (tests/test_arith_emu.py propagates these bounds in exact rationals).
+constexpr u64 F64_PMAX = 9ull << 47;
 struct F64Mod {
   double p, ip;   // p, 1/p
 };
@@ -232,7 +241,7 @@ __host__ __device__ __forceinline__ double f64_recip_near(double ib, double fd, 
 // The digits of r = a + b p as a ladder leaves them: |a| <= p/2 (1 + 2^-20) after a window
 // step (0 <= a < p if none ran) and |b| <= 1.1 p.  Then r = 1 (mod p), the FLT condition,
 // holds exactly when a == 1, and q = (r - 1) / p = b mod p takes at most two corrections.
-// q < 2^50 is read from the bits of q + 2^52 (exponent fixed, the mantissa is q).  Returns
+// q < 2^51 is read from the bits of q + 2^52 (exponent fixed, the mantissa is q).  Returns
 // false, leaving q alone, when a != 1.
 __device__ __forceinline__ bool f64_quotient(double a, double b, double pd, u64& q) {
   if (a != 1.0) return false;

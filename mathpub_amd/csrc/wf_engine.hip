@@ -842,7 +842,7 @@ __global__ void __launch_bounds__(PROVE_TILE) k_prove_gather(const SegDesc* __re
 #define TEST_TILE 1024   // primes per test work item (4 rounds of 256 lanes)
 #endif
 
-// arithmetic classes: FP64 p-adic (p < 2^50), 32-bit Montgomery with 2 / 3 / 4 limbs,
+// arithmetic classes: FP64 p-adic (p < F64_PMAX = 9 * 2^47), 32-bit Montgomery with 2 / 3 / 4 limbs,
 // 4 limbs keeping the carry bit (p >= 2^63), radix-2^28 Montgomery (p >= 2^47)
 enum Arith { A_F64 = 0, A_M2, A_M3, A_M4, A_FULL, A_M28, A_NCLS };
 
@@ -1063,7 +1063,7 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
       // result is not accounted.
       const bool big = d.base >= (1ull << 32);
       const bool near = d.base >= (1ull << 40);
-      const double based = (double)d.base;     // p = base + offset < 2^50: exact as a double
+      const double based = (double)d.base;     // p = base + offset < F64_PMAX: exact as a double
       const double ib = near ? 1.0 / based : 0.0;
       for (u32 base = beg; base < end; base += 2 * blockDim.x) {
         const u32 i0 = base + threadIdx.x, i1 = i0 + blockDim.x;
@@ -1121,7 +1121,7 @@ __global__ void __launch_bounds__(256) k_test(const SegDesc* __restrict__ segs, 
   if (cur != 0xFFFFFFFFu) flush_sums(o, cur, curj, s);
 }
 
-// Composite-base mode (p < 2^50): the quotients of the ladder primes l_s first, two ladders
+// Composite-base mode (p < F64_PMAX): the quotients of the ladder primes l_s first, two ladders
 // per lane on the lane's prime (fp64_powc_x2 with two power tables, window W for all), kept
 // in LDS; then for each base j, q_j = sum of e_js q_s (mod p), since q_p(ab) = q_p(a) +
 // q_p(b) (mod p) for p not dividing ab.  A pair whose ladder residue is not 1 mod p (not
@@ -1177,7 +1177,7 @@ __global__ void __launch_bounds__(256) k_test_derived(const SegDesc* __restrict_
     const bool big = d.base >= (1ull << 32);
     const u64 idle = big ? 4294967311ull : 3ull;
     const i64 nT = o.nearv ? o.nearv[d.rid] : o.near_T;
-    const double based = (double)d.base;          // p = base + offset < 2^50: exact as a double
+    const double based = (double)d.base;          // p = base + offset < F64_PMAX: exact as a double
     for (u32 base = beg; base < end; base += blockDim.x) {
       const u32 i = base + threadIdx.x;
       const bool act = i < end;
@@ -1248,7 +1248,7 @@ __global__ void __launch_bounds__(256) k_test_derived(const SegDesc* __restrict_
 
 // Quotients of arbitrary odd p (check / selftest) for the base in slot `slot` of the base
 // tables: per-lane choice of the path, the scan's classes (int32 = 1 selects the 32-bit
-// Montgomery ladders above 2^50).
+// Montgomery ladders from F64_PMAX up).
 __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u64* __restrict__ rhi_out,
                        u64* __restrict__ rlo_out, u64 n, int int32, u32 slot) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1259,7 +1259,7 @@ __global__ void k_quot(const u64* __restrict__ p_in, u64* __restrict__ q_out, u6
   const int w = (int)c_bwin[slot];
   const u32* pw = c_bpow + slot * BPOW;
   u64 rhi, rlo;
-  if (p < (1ull << 50)) residue<A_F64>(p, eb, b, w, pw, rhi, rlo);
+  if (p < F64_PMAX) residue<A_F64>(p, eb, b, w, pw, rhi, rlo);
   else if (!int32) residue<A_M28>(p, eb, b, w, pw, rhi, rlo);
   else if (p < (1ull << 63)) residue<A_M4>(p, eb, b, w, pw, rhi, rlo);
   else residue<A_FULL>(p, eb, b, w, pw, rhi, rlo);
@@ -1889,7 +1889,7 @@ extern "C" int wf_info(wf_engine* e, char* buf, int len) {
   if (!e || !buf || len <= 0) return WF_E_ARG;
   snprintf(buf, (size_t)len, "%s (%s), %d CUs, LDS segment %u B (%zu B per workgroup), %s", e->prop.name,
            e->prop.gcnArchName, e->prop.multiProcessorCount, e->cfg.seg_bytes, (size_t)e->prop.sharedMemPerBlock,
-           e->cfg.arith == 1 ? "32-bit Montgomery" : "FP64 p-adic below 2^50, radix-2^28 Montgomery above");
+           e->cfg.arith == 1 ? "32-bit Montgomery" : "FP64 p-adic below 9*2^47, radix-2^28 Montgomery above");
   return WF_OK;
 }
 
@@ -1994,7 +1994,7 @@ static void split_classes(const wf_engine* e, u64 lo, u64 hi, u32 rid, std::vect
   if (e->cfg.arith == 1) {
     cuts = {{1ull << 31, A_M2}, {1ull << 47, A_M3}, {1ull << 63, A_M4}, {0, A_FULL}};
   } else {
-    cuts = {{1ull << 50, A_F64}, {0, A_M28}};
+    cuts = {{F64_PMAX, A_F64}, {0, A_M28}};
   }
   u64 cur = lo;
   for (auto& c : cuts) {

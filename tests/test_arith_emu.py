@@ -49,6 +49,16 @@ def call(fn, *args):
     return (hi.value << 64) | lo.value
 
 
+F64_PMAX = 9 << 47   # the FP64 p-adic class bound of wf_arith.h
+
+
+def top_of_class(n, seed):
+    """Odd numbers from 2^50 to the FP64 class bound, with both ends."""
+    rng = random.Random(seed)
+    return [(1 << 50) + 1, (1 << 50) + 27, F64_PMAX - 1, F64_PMAX - 3, 12 * 10 ** 14 + 7] + \
+        [rng.randrange(1 << 50, F64_PMAX) | 1 for _ in range(n)]
+
+
 def samples(lo_bits, hi_bits, n, seed):
     rng = random.Random(seed)
     out = []
@@ -112,6 +122,49 @@ def test_fp64_two_ladders_big(emu):
         for e in (eb, min(50, eb + rng.randint(0, 7))):
             emu.emu_fp64x2(p0, p1, e, out, 1)
             check_pair(out, 2, p0, p1, e)
+
+
+def test_fp64_top_of_class(emu):
+    """2^50 <= p < 9 * 2^47, where the quotient errors are larger than below 2^50 (the bounds
+    hold over the five squarings between window steps): the single ladder, the two-ladder
+    form with both window steps, paired with smaller big p."""
+    ps = top_of_class(6000, 50)
+    small = samples(33, 50, 200, 51)
+    rng = random.Random(52)
+    out = (ctypes.c_uint64 * 8)()
+    for p0 in ps:
+        assert call(emu.emu_fp64, p0) == pow(2, p0 - 1, p0 * p0), p0
+        p1 = rng.choice(ps + small) | 1
+        eb = max((p0 - 1).bit_length(), (p1 - 1).bit_length())
+        for big in (0, 1):
+            emu.emu_fp64x2(p0, p1, eb, out, big)
+            check_pair(out, 2, p0, p1, eb)
+
+
+def test_fp64_bounds_to_the_class_top():
+    """The error model of wf_arith.h's FP64 ladder, propagated in exact rationals: from the
+    bounds a window step leaves (|a| <= p/2 (1 + 2^-20), |b| <= 1.1 p), five squarings at
+    p = 9 * 2^47 keep every rounded quotient inside round_mul's range (|x y| < 2^51) and every
+    digit below 2^53; at 1.25 * 2^50 the margin is smaller but still there, and the model
+    fails before 2^51 (so it is not vacuous)."""
+    from fractions import Fraction as Fr
+    u = Fr(1, 1 << 53)
+    rel = u + u * (1 + Fr(1, 128)) + u * u * (1 + Fr(1, 128))   # fl(x) * ip against x / p
+
+    def worst(p):
+        a, b, w = Fr(1, 2) * (1 + Fr(1, 1 << 20)), Fr(11, 10), Fr(0)
+        for _ in range(5):
+            # |h1 ip| <= a^2 p (1 + rel), |h2 ip| <= a b p (1 + rel), against 2^51
+            w = max(w, a * a * p * (1 + rel) / (1 << 51), a * b * p * (1 + rel) / (1 << 51))
+            k1err = Fr(1, 2) + a * a * p * rel          # |k1 - a^2/p|
+            r = Fr(1, 2) + a * b * p * rel              # |r2| / p
+            a, b = k1err, a * a + k1err / p + 2 * r
+            w = max(w, b * p / (1 << 53))
+        return w
+    assert worst(Fr(1 << 50)) < Fr(56, 100)
+    assert worst(Fr(F64_PMAX)) < Fr(69, 100)
+    assert worst(Fr(5 << 48)) < 1
+    assert worst(Fr(1 << 51)) > 1
 
 
 @pytest.mark.parametrize("L,full,lo,hi", [(2, 0, 2, 31), (3, 0, 32, 47), (4, 0, 48, 63), (4, 1, 64, 64),
@@ -381,6 +434,20 @@ def test_fp64_any_base_big(emu):
             check_pair(out, b, p0, p1, eb)
 
 
+def test_fp64_any_base_top_of_class(emu):
+    """Every base through the FP64 two-ladder form with both window steps, 2^50 <= p < 9 * 2^47."""
+    rng = random.Random(53)
+    ps = top_of_class(150, 54)
+    out = (ctypes.c_uint64 * 8)()
+    for b in BASES:
+        for p0 in ps:
+            p1 = rng.choice(ps)
+            eb = max((p0 - 1).bit_length(), (p1 - 1).bit_length())
+            for big in (0, 1):
+                emu.emu_fp64c(b, p0, p1, eb, out, big)
+                check_pair(out, b, p0, p1, eb)
+
+
 def test_radix28_any_base(emu):
     """Any base through the radix-2^28 ladder with small-constant multiplication, 2^50 <= p < 2^64."""
     ps = [(1 << 50) + 27, (1 << 63) - 25, (1 << 63) + 29, (1 << 64) - 59, (1 << 64) - 1, 10 ** 18 + 9] + \
@@ -603,13 +670,14 @@ def test_neg_mod_fp_random(emu):
 def test_recip_near_bound(emu):
     """f64_recip_near: 1/p from the segment's 1/base (base >= 2^40, offset f < 2^22) within
     2^-53 (1 + 2^-7) of 1/p, relative, checked exactly with rationals; the edges are the
-    smallest base, the largest offset and p just below 2^50."""
+    smallest base, the largest offset and p just below the FP64 class bound 9 * 2^47."""
     from fractions import Fraction
     rng = random.Random(7)
     cases = [((1 << 40), (1 << 22) - 1), ((1 << 40), 1), ((1 << 50) - (1 << 22), (1 << 22) - 1),
-             (10 ** 15, 0), (10 ** 15, 1 << 21), (30 * ((1 << 45) // 30), 12345)]
+             (10 ** 15, 0), (10 ** 15, 1 << 21), (30 * ((1 << 45) // 30), 12345),
+             (30 * ((F64_PMAX - (1 << 22)) // 30), (1 << 22) - 1)]
     for _ in range(20000):
-        base = 30 * (rng.randrange(1 << 40, (1 << 50) - (1 << 22)) // 30)
+        base = 30 * (rng.randrange(1 << 40, F64_PMAX - (1 << 22)) // 30)
         if base < 1 << 40:
             base += 30
         cases.append((base, rng.randrange(0, 1 << 22)))
@@ -626,10 +694,12 @@ def test_recip_near_bound(emu):
 
 def test_fp64_two_ladders_near(emu):
     """The kernel's base-2 ladder pair above 2^40 with 1/p from f64_recip_near, on primes and
-    odd composites of segments at 2^40, 1e15 and just below 2^50, against Python's pow."""
+    odd composites of segments at 2^40, 1e15, just below 2^50, at 1.2e15 and just below the
+    FP64 class bound, against Python's pow."""
     rng = random.Random(11)
     out = (ctypes.c_uint64 * 8)()
-    for base in (30 * ((1 << 40) // 30 + 1), 10 ** 15, 30 * (((1 << 50) - (1 << 22)) // 30)):
+    for base in (30 * ((1 << 40) // 30 + 1), 10 ** 15, 30 * (((1 << 50) - (1 << 22)) // 30), 12 * 10 ** 14,
+                 30 * ((F64_PMAX - (1 << 22)) // 30)):
         eb = (base + (1 << 22)).bit_length()
         for _ in range(300):
             f0, f1 = rng.randrange(1, 1 << 22, 2), rng.randrange(1, 1 << 22, 2)

@@ -33,8 +33,8 @@ def corrupted(p, b):
     return ((r - 1) * pow(p, -1, 1 << 64)) & M64, r
 
 
-# (arith, bases, window around the faulted prime): FP64 p-adic below 2^50 (base 2 and the
-# generic window ladder), composite bases derived from prime ladders, radix 2^28 above 2^50,
+# (arith, bases, window around the faulted prime): FP64 p-adic below 9 * 2^47 (base 2 and the
+# generic window ladder), composite bases derived from prime ladders, radix 2^28 above it,
 # and the 32-bit limb ladders (arith = 1) below and above 2^63
 CASES = [(0, [2], 10 ** 15), (0, [3], 10 ** 15), (0, [2, 6, 10, 12], 10 ** 15), (0, [2], 3 * 10 ** 4),
          (0, [2], 10 ** 18), (0, [7], 10 ** 18), (1, [2], 10 ** 15), (1, [2], (1 << 63) + (1 << 40))]

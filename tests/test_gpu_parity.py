@@ -19,6 +19,8 @@ WINS = json.load(open(os.path.join(G, "base2_windows.json")))
 FACTS = json.load(open(os.path.join(G, "reference_facts.json")))
 
 
+F64_PMAX = 9 << 47   # where the engine's FP64 p-adic class ends and the radix-2^28 class begins
+
 def say(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -45,7 +47,7 @@ def test_primes_match_oracle(eng, lo, hi):
 def test_quotients_vs_pow(eng):
     rng = random.Random(0x243F6A8885A308D3)
     ps = [3, 5, 7, 9, 15, 1093, 3511, (1 << 50) - 27, (1 << 50) + 55, (1 << 63) - 25, (1 << 63) + 29,
-          18446744073709551557, (1 << 64) - 1]
+          18446744073709551557, (1 << 64) - 1, F64_PMAX - 1, F64_PMAX - 3, F64_PMAX + 1, 12 * 10 ** 14 + 7]
     for _ in range(4000):
         bits = rng.randint(2, 64)
         p = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
@@ -78,6 +80,7 @@ def test_golden_windows(eng, w):
 
 @pytest.mark.parametrize("lo,hi", [(10 ** 15, 10 ** 15 + 2 * 10 ** 8), (10 ** 12 + 12345, 10 ** 12 + 10 ** 8 + 7),
                                    ((1 << 50) - 10 ** 7, (1 << 50) + 10 ** 7), (3, 10 ** 8),
+                                   (F64_PMAX - 10 ** 7, F64_PMAX + 10 ** 7), (12 * 10 ** 14, 12 * 10 ** 14 + 10 ** 8),
                                    ((1 << 32) - 3 * 10 ** 6, (1 << 32) + 3 * 10 ** 6)])
 def test_scan_vs_oracle(eng, lo, hi):
     say("scan vs oracle", lo, hi)
@@ -103,7 +106,8 @@ def test_batch_equals_single(eng):
     for _ in range(40):
         lo = rng.randrange(10 ** 9, 10 ** 18)
         ranges.append((lo, lo + rng.randrange(1, 3 * 10 ** 6)))
-    ranges += [(0, 100), (1 << 63, (1 << 63) + 10 ** 6), ((1 << 64) - 10 ** 6, 1 << 64), ((1 << 50) - 5, (1 << 50) + 5)]
+    ranges += [(0, 100), (1 << 63, (1 << 63) + 10 ** 6), ((1 << 64) - 10 ** 6, 1 << 64), ((1 << 50) - 5, (1 << 50) + 5),
+               (F64_PMAX - 5, F64_PMAX + 5)]
     say("batch", len(ranges))
     res, ev = eng.scan_batch(ranges, near_T=0)
     for (lo, hi), r in zip(ranges, res):
@@ -263,7 +267,7 @@ def oracle_scan(lo, hi, bases, near_T):
 @pytest.mark.parametrize("w", range(3))
 def test_ckwin_17_bases_reference_data(feng, w):
     """The reference's own checksum windows (wieferich_cks.h, generated by verify_wieferich.py)
-    for its 17 default bases: FP64 below 2^50, radix-2^28 above."""
+    for its 17 default bases: FP64 below 9 * 2^47, radix-2^28 above."""
     c = FACTS["ckwin"][w]
     say("ckwin", c["lo"], c["hi"])
     feng.set_bases(OEIS)
@@ -288,6 +292,7 @@ def test_known_solutions_all_bases(feng):
 @pytest.mark.parametrize("lo,hi", [(3, 2 * 10 ** 6), (10 ** 12, 10 ** 12 + 2 * 10 ** 7),
                                    ((1 << 32) - 2 * 10 ** 6, (1 << 32) + 2 * 10 ** 6),
                                    ((1 << 50) - 5 * 10 ** 6, (1 << 50) + 5 * 10 ** 6),
+                                   (F64_PMAX - 5 * 10 ** 6, F64_PMAX + 5 * 10 ** 6),
                                    ((1 << 63) - 5 * 10 ** 6, (1 << 63) + 5 * 10 ** 6), ((1 << 64) - 10 ** 7, 1 << 64)])
 def test_any_bases_vs_oracle(feng, lo, hi):
     """Windows 5..1 (bases up to 2^32 - 1), p | b pairs, near lists and sums of q per base."""
@@ -331,14 +336,15 @@ def test_batch_any_bases_equals_single(feng):
 def test_composite_bases_derived_equal_direct(bases):
     """Composite bases from their prime factors' quotients (the default whenever it needs
     fewer ladders) against one ladder per base: bit-exact totals, sums and events, from p = 3
-    (p dividing ladder primes and bases) across 2^32 to the FP64 class end at 2^50, single
-    and batched."""
+    (p dividing ladder primes and bases) across 2^32 and 2^50 to the FP64 class end at
+    9 * 2^47, single and batched."""
     auto, off = wf.Engine(), wf.Engine(derive=wf.DERIVE_OFF)
     try:
         for e in (auto, off):
             e.set_bases(bases)
         for lo, hi in [(3, 3 * 10 ** 6), (10 ** 12 + 1, 10 ** 12 + 3 * 10 ** 6),
-                       ((1 << 32) - 10 ** 6, (1 << 32) + 10 ** 6), ((1 << 50) - 2 * 10 ** 6, (1 << 50) + 10 ** 6)]:
+                       ((1 << 32) - 10 ** 6, (1 << 32) + 10 ** 6), ((1 << 50) - 2 * 10 ** 6, (1 << 50) + 10 ** 6),
+                       (F64_PMAX - 2 * 10 ** 6, F64_PMAX + 10 ** 6)]:
             say("derived vs direct", bases[:4], lo, hi)
             a = auto.scan(lo, hi, near_T=1000)
             b = off.scan(lo, hi, near_T=1000)
@@ -346,7 +352,7 @@ def test_composite_bases_derived_equal_direct(bases):
                 (b["primes"], b["tests"], b["near"], b["flt"], b["cks_all"], b["sumq_all"], b["nev"]), (lo, hi)
             assert a["events"] == b["events"]
         rng = random.Random(9)
-        ranges = [(lo, lo + rng.randrange(1, 10 ** 6)) for lo in (rng.randrange(10 ** 6, 1 << 50) for _ in range(40))]
+        ranges = [(lo, lo + rng.randrange(1, 10 ** 6)) for lo in (rng.randrange(10 ** 6, F64_PMAX) for _ in range(40))]
         pa, ea = auto.scan_batch(ranges, near_T=1000)
         pb, eb = off.scan_batch(ranges, near_T=1000)
         assert [(x["tests"], x["cks_all"], x["sumq_all"]) for x in pa] == [(x["tests"], x["cks_all"], x["sumq_all"])
