@@ -1017,7 +1017,8 @@ __device__ __forceinline__ void residue(u64 p, int eb, u32 b, int w, const u32* 
 // items in turn and keep their sums until the (range, base) changes (uniform across the
 // workgroup).  The bases' power tables are copied to LDS once per workgroup.
 #ifndef M28_WAVES_EU
-#define M28_WAVES_EU 6   // the radix-2^28 test kernel: 6 wavefronts per SIMD (<= 80 VGPRs)
+#define M28_WAVES_EU 8   // the radix-2^28 test kernel: 8 wavefronts per SIMD (<= 64 VGPRs; the
+                         // ladder stays in registers, only its set-up spills a few values)
 #endif
 #ifndef F64_WAVES_EU
 #define F64_WAVES_EU 6   // the FP64 test kernel: 6 wavefronts per SIMD (<= 80 VGPRs)
