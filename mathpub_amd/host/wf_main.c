@@ -770,6 +770,7 @@ static int selftest_arith(void)
 {
     enum { NQ = 8000 };
     static const u64 edge[] = {3, 5, 7, 1093, 3511, 4294967291ULL, 4294967311ULL, (1ULL << 50) - 27,
+                               (1ULL << 50) + 27, (9ULL << 47) - 1, (9ULL << 47) + 1,   /* FP64 class end */
                                (1ULL << 63) - 25, (1ULL << 63) + 29, 18446744073709551557ULL, UINT64_MAX};
     static const u32 tb[] = {2, 3, 30, 1625, 4294967291U};
     u64 *p = malloc(4 * NQ * sizeof *p);
