@@ -545,8 +545,8 @@ __global__ void __launch_bounds__(1024) k_sieve(const SegDesc* __restrict__ segs
           if (w + k * bd < nwords) seg[w + k * bd] = v[k];
       }
     }
-    __syncthreads();
-    if (d.base == 0 && threadIdx.x == 0) seg[0] &= ~0x3Eu;   // 7, 11, 13, 17, 19 themselves are prime
+    // 7, 11, 13, 17, 19 themselves are prime; thread 0 stored word 0 itself just above
+    if (d.base == 0 && threadIdx.x == 0) seg[0] &= ~0x3Eu;
     __syncthreads();
     const u32 nsmall = min(d.nq, q1idx);
     // Placement (uniform per segment): below 2^53 from the FP64 reciprocal table, else by
